@@ -1,0 +1,162 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and, when it has been
+built, of the compiled reference (oracle/_ref/libimop_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.
+
+The C restatement lives in imop_oracle.c; every function there cites the
+reference file:line it follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libimop_ref.so")
+
+HQLI, MG, PRISM = 1, 2, 8
+RGGB, BGGR, GRBG, GBRG = 0, 1, 2, 3
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and the reference library when its sources are
+    present) with oracle/Makefile / oracle/build_ref.sh."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run([os.path.join(HERE, "build_ref.sh")], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u8p)
+
+
+class _Lib:
+    def __init__(self, path: str, prefix: str):
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle() -> ctypes.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.orc_demosaic.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 6
+        lib.orc_pipeline.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 6 + [_fp, ctypes.c_int]
+        lib.orc_black_level.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_int]
+        lib.orc_packed_pixel.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t]
+        lib.orc_unpack.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_u8p]
+        lib.orc_packed_bytes.restype = ctypes.c_size_t
+        lib.orc_packed_bytes.argtypes = [ctypes.c_int] * 3
+        lib.orc_set_mg8_scalar.argtypes = [ctypes.c_int]
+        lib.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _oracle = lib
+    return _oracle
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def reference() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_PATH)
+        lib.ref_demosaic_cpu.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 6
+        lib.ref_black_level.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_int]
+        lib.ref_get_packed_raw.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_size_t, ctypes.c_int]
+        lib.ref_get_raw.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_size_t, ctypes.c_int]
+        lib.ref_set_num_threads.argtypes = [ctypes.c_int]
+        _ref = lib
+    return _ref
+
+
+def rgb_bytes(w: int, h: int, bd: int) -> int:
+    return w * h * 3 * (2 if bd > 8 else 1)
+
+
+def _as_rgb(buf: np.ndarray, w: int, h: int, bd: int) -> np.ndarray:
+    if bd > 8:
+        return buf.view(np.uint16).reshape(h, w, 3)
+    return buf.reshape(h, w, 3)
+
+
+def demosaic(raw: np.ndarray, w: int, h: int, pattern: int, algo: int, bd: int,
+             packed: bool = False, black_level=None, per_channel: bool = False,
+             out: np.ndarray | None = None) -> tuple[int, np.ndarray]:
+    """Oracle demosaic of a byte buffer; returns (error_code, rgb array).
+
+    black_level None -> demosaic only; else unpack -> black level -> demosaic.
+    The output buffer starts zeroed (the reference's corner rule copies the
+    caller's prior bytes, and zero is the convention every reference test
+    uses)."""
+    lib = oracle()
+    raw = np.ascontiguousarray(raw).view(np.uint8).ravel()
+    if out is None:
+        out = np.zeros(rgb_bytes(w, h, bd), dtype=np.uint8)
+    if black_level is None:
+        e = lib.orc_demosaic(_ptr(raw), _ptr(out), w, h, pattern, algo, bd, int(packed))
+    else:
+        bl = np.asarray(black_level, dtype=np.float32)
+        if bl.size == 1:
+            bl = np.repeat(bl.reshape(1), 4)
+        bl = np.ascontiguousarray(bl, dtype=np.float32)
+        e = lib.orc_pipeline(_ptr(raw), _ptr(out), w, h, pattern, algo, bd, int(packed),
+                             bl.ctypes.data_as(_fp), int(per_channel))
+    return e, _as_rgb(out, w, h, bd)
+
+
+def ref_demosaic(raw: np.ndarray, w: int, h: int, pattern: int, algo: int, bd: int,
+                 packed: bool = False, black_level=None, per_channel: bool = False
+                 ) -> tuple[int, np.ndarray]:
+    """The compiled reference, composed the same way as `demosaic`."""
+    lib = reference()
+    raw = np.ascontiguousarray(raw).view(np.uint8).ravel().copy()
+    out = np.zeros(rgb_bytes(w, h, bd), dtype=np.uint8)
+    if black_level is not None:
+        bl = np.asarray(black_level, dtype=np.float32)
+        if bl.size == 1:
+            bl = np.repeat(bl.reshape(1), 4)
+        bl = np.ascontiguousarray(bl, dtype=np.float32)
+        if packed:
+            n = w * h
+            un = np.zeros(n * (2 if bd > 8 else 1), dtype=np.uint8)
+            vals = np.array([lib.ref_get_raw(_ptr(raw), i % w, i // w, w, bd, 1, 0, h)
+                             for i in range(n)], dtype=np.uint16 if bd > 8 else np.uint8)
+            un[:] = vals.view(np.uint8)
+            raw = un
+        e = lib.ref_black_level(_ptr(raw), w, h, pattern, bd, bl.ctypes.data_as(_fp),
+                                int(per_channel))
+        if e != 0:
+            return e, _as_rgb(out, w, h, bd)
+        packed = False
+    e = lib.ref_demosaic_cpu(_ptr(raw), _ptr(out), w, h, pattern, algo, bd, int(packed))
+    return e, _as_rgb(out, w, h, bd)
+
+
+def unpack(raw: np.ndarray, w: int, h: int, bd: int, packed: bool) -> np.ndarray:
+    """Oracle unpack to u8 (bd <= 8) or u16 values, shape (h, w)."""
+    lib = oracle()
+    raw = np.ascontiguousarray(raw).view(np.uint8).ravel()
+    out = np.zeros(w * h * (2 if bd > 8 else 1), dtype=np.uint8)
+    lib.orc_unpack(_ptr(raw), w, h, bd, int(packed), _ptr(out))
+    return (out.view(np.uint16) if bd > 8 else out).reshape(h, w)
