@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 Bayer demosaic path on one MI355X.
+
+Default workload: one 4024x3032 RGGB frame, 12-bit packed (2 px / 3 B),
+black level 64, MG ("Malvar-He-Cutler") demosaic -> u16 interleaved RGB,
+fused in one kernel launch per frame.  A step is one such frame.  Inputs are
+seeded synthetic scenes already resident in HBM; six input/output buffer sets
+(549 MB, above the 256 MB Infinity Cache) are rotated so every step reads
+and writes HBM.
+
+Prints one JSON line: Mpix/s (whole-job), the HBM roofline of the kernel
+(algorithmic bytes = packed input read once + RGB written once, divided by
+the mean launch duration measured with HIP events on the launch stream), and
+the reference's CPU path timed on this host (oracle/_ref, the reference
+compiled from its own sources; the oracle restatement when that is absent).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "im-operator_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (w, h, bd, packed, algo, black level, seed, diagonal lines)
+    "mg12": (4024, 3032, 12, True, 2, 64.0, 2, 0),
+    "prism10": (4024, 3032, 10, True, 8, 64.0, 3, 200),
+    "hqli8": (1006, 758, 8, False, 1, None, 1, 0),
+    "hqli12": (4024, 3032, 12, True, 1, 64.0, 2, 0),
+}
+
+METRIC = "Mpix/s + %HBM roofline, 4024x3032 RAW12 MHC demosaic; batched & fwd+bwd Mpix/s"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def golden_sha(name):
+    """SHA-256 of the reference's output for this workload, if recorded."""
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[name]
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+            for c in json.load(f)["cases"]:
+                if (c["kind"] == "scene" and c["w"] == w and c["h"] == h and c["algo"] == algo and c["bd"] == bd
+                        and c["seed"] == seed):
+                    return c["sha256"]
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(raw, name, budget_s=12.0):
+    """The reference CPU path on this host's cores: unpack (pixel::get_raw) ->
+    black_level::process_global -> imop::demosaic_cpu, as one frame of work;
+    repeated for about budget_s seconds, median frame time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[name]
+    blv = None if bl is None else [bl] * 4
+    if O.reference_available():
+        fn, kind, threads = O.ref_demosaic, "reference", O.reference().ref_num_threads()
+    else:
+        fn, kind, threads = O.demosaic, "port", O.oracle().orc_num_threads()
+    times = []
+    t_end = time.time() + budget_s
+    while len(times) < 3 or (time.time() < t_end and len(times) < 200):
+        t0 = time.perf_counter()
+        e, _ = fn(raw, w, h, 0, algo, bd, packed, blv)
+        times.append(time.perf_counter() - t0)
+        if e != 0:
+            raise RuntimeError(f"CPU baseline returned {e}")
+    times.sort()
+    med = times[len(times) // 2]
+    return {
+        "value": round(w * h / med / 1e6, 2),
+        "unit": "Mpix/s",
+        "cores": int(threads),
+        "kind": kind,
+        "sample": f"{len(times)} full {w}x{h} frames of the same workload, median frame {med * 1e3:.1f} ms "
+                  f"(OpenMP rows, {threads} threads; unpack and black level single-threaded as in the reference)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="mg12", choices=sorted(WORKLOADS))
+    ap.add_argument("--sets", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(frames are independent: N GPUs would each run this same per-frame kernel).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[args.workload]
+    blv = None if bl is None else [bl] * 4
+    log(f"bench: workload {args.workload} {w}x{h} bd{bd} packed={packed} algo={algo} bl={bl}")
+    vals = scene.make_scene(h, w, bd, 0, seed=seed, n_lines=lines)
+    raw = scene.raw_bytes(vals, bd, packed)
+    in_b = raw.size
+    out_b = imop_amd.rgb_byte_size(w, h, bd)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(args.sets)]
+    d_out = [torch.empty(out_b, dtype=torch.uint8, device=dev) for _ in range(args.sets)]
+    stream = torch.cuda.current_stream()
+
+    def launch(k):
+        e = imop_amd.demosaic(d_in[k % args.sets], d_out[k % args.sets], w, h, 0, algo, bd, packed, blv,
+                              stream=stream)
+        if e != 0:
+            raise RuntimeError(f"imop_hip_demosaic returned {e}")
+
+    # correctness of the measured configuration against the golden digest
+    launch(0)
+    torch.cuda.synchronize()
+    want = golden_sha(args.workload)
+    got = hashlib.sha256(d_out[0].cpu().numpy().tobytes()).hexdigest()
+    parity = "unchecked" if want is None else ("bit-exact vs reference digest" if got == want else "MISMATCH")
+    log(f"bench: parity {parity}")
+
+    for k in range(args.warmup):
+        launch(k)
+    torch.cuda.synchronize()
+    log(f"bench: warmup {args.warmup} done")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        launch(k)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    total_ms = starts[0].elapsed_time(ends[-1])
+    mean_launch_ms = sum(per_launch_ms) / len(per_launch_ms)
+    ms_per_step = total_ms / args.steps
+    px = w * h
+    value = px * args.steps / (total_ms / 1e3) / 1e6
+    alg_bytes = in_b + out_b
+    achieved = alg_bytes / (mean_launch_ms / 1e3) / 1e9
+    log(f"bench: {args.steps} steps in {total_ms:.3f} ms (wall {wall * 1e3:.1f} ms); mean launch {mean_launch_ms * 1e3:.2f} us")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": {"mg12": "MG demosaic 4024x3032 RGGB RAW12-packed, black level 64 -> u16 RGB",
+                         "prism10": "PRISM demosaic 4024x3032 RGGB RAW10-packed, black level 64 -> u16 RGB",
+                         "hqli8": "HQLI (bilinear) 1006x758 RGGB u8 -> u8 RGB",
+                         "hqli12": "HQLI (bilinear) 4024x3032 RGGB RAW12-packed, black level 64 -> u16 RGB"}[args.workload],
+            "frames_per_step": 1,
+            "width": w, "height": h, "bit_depth": bd, "packed": packed, "black_level": bl,
+            "buffer_sets": args.sets,
+            "parity": parity,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "mean_launch_us": round(mean_launch_ms * 1e3, 3),
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(raw, args.workload)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

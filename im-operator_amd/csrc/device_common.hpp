@@ -1,0 +1,120 @@
+// device_common.hpp -- frame parameters, raw-pixel decode and black level on
+// gfx950.  Everything here works on one sample at a time; the tiled kernels
+// use it for their LDS fill and the border paths use it directly against HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imop_hip {
+
+// Raw layouts the decoder understands (include/imop/pixel_utils.hpp:58-189).
+enum Fmt : int {
+    FMT_U8 = 0,    // one byte per pixel (bd <= 8, packed or not: get_raw reads bytes)
+    FMT_U16 = 1,   // little-endian u16 per pixel, MSB-aligned samples shifted down
+    FMT_P12 = 2,   // 2 px / 3 B
+    FMT_P10 = 3,   // 4 px / 5 B
+    FMT_PGEN = 4,  // LSB-first bitstream, any other depth
+};
+
+// Demosaic modes of the tiled kernels.
+enum Mode : int {
+    MODE_HQLI = 0,    // bilinear, integer averages (hqli.cpp)
+    MODE_MG_EXACT = 1,// MG float path, reproduced exactly in units of 1/32 (mg.cpp:163-216)
+    MODE_MG_INT8 = 2, // MG 8-bit integer path, AVX2 variant (avx2.cpp:422-595)
+};
+
+// Everything a kernel needs about one frame.  Plain data; lives in kernel
+// arguments (plain launches) or in a device-side table (batched launches).
+struct FrameParams {
+    const uint8_t* in;    // raw Bayer bytes (device)
+    uint8_t* out;         // interleaved RGB, u8 if bd <= 8 else u16 (device)
+    uint64_t in_bytes;    // size of the packed stream / unpacked buffer
+    int32_t w, h;
+    int32_t bd;
+    int32_t fmt;          // Fmt
+    int32_t maxv;         // (1 << bd) - 1
+    int32_t clamp_hi;     // final clamp of the integer paths
+    int32_t bl_on;        // black level applied before demosaic
+    int32_t pattern;      // BayerPattern, 0..3
+    float bl_q[4];        // offset by (y & 1) * 2 + (x & 1), already max(0, .)
+};
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return imin(imax(v, lo), hi); }
+
+// Packed 12-bit: pixel i of a 3-byte pair group.  get_packed_raw's bound
+// (pixel_utils.hpp:69-80): a group touching the stream's last byte reads 0.
+__device__ __forceinline__ int decode_p12(const uint8_t* __restrict__ d, uint64_t i, uint64_t nb) {
+    uint64_t o = (i >> 1) * 3u;
+    if (o + 2u >= nb) return 0;
+    unsigned b1 = d[o + 1];
+    if ((i & 1u) == 0) return (int)(((unsigned)d[o] << 4) | (b1 & 0xFu));
+    return (int)(((unsigned)d[o + 2] << 4) | (b1 >> 4));
+}
+
+// Packed 10-bit (pixel_utils.hpp:82-98): four high bytes, then the tails.
+__device__ __forceinline__ int decode_p10(const uint8_t* __restrict__ d, uint64_t i, uint64_t nb) {
+    uint64_t o = (i >> 2) * 5u;
+    if (o + 4u >= nb) return 0;
+    unsigned s = (unsigned)(i & 3u);
+    return (int)(((unsigned)d[o + s] << 2) | (((unsigned)d[o + 4] >> (2u * s)) & 3u));
+}
+
+// LSB-first bitstream (pixel_utils.hpp:100-117).
+__device__ __forceinline__ int decode_pgen(const uint8_t* __restrict__ d, uint64_t i, uint64_t nb, int bd) {
+    uint64_t bit = i * (uint64_t)bd;
+    uint64_t o = bit >> 3;
+    int sh = (int)(bit & 7u);
+    if (o >= nb) return 0;
+    int n = (bd + sh + 7) >> 3;
+    if ((uint64_t)n > nb - o) n = (int)(nb - o);
+    uint32_t v = 0;
+    for (int k = 0; k < n; ++k) v |= (uint32_t)d[o + k] << (8 * k);
+    return (int)((v >> sh) & ((1u << bd) - 1u));
+}
+
+// One raw sample as the demosaic sees it: decoded, then either black level
+// (unpack -> process_global / process_per_channel -> demosaic of the
+// unpacked buffer; black_level/global.cpp:6-24) or the u16 MSB-alignment of
+// pixel::get_raw (pixel_utils.hpp:129-150).
+__device__ __forceinline__ int sample_at(const FrameParams& f, int x, int y) {
+    const uint64_t i = (uint64_t)y * (uint64_t)f.w + (uint64_t)x;
+    int v;
+    switch (f.fmt) {
+    case FMT_U8: v = f.in[i]; break;
+    case FMT_U16: v = (int)((const uint16_t*)f.in)[i]; break;
+    case FMT_P12: v = decode_p12(f.in, i, f.in_bytes); break;
+    case FMT_P10: v = decode_p10(f.in, i, f.in_bytes); break;
+    default: v = decode_pgen(f.in, i, f.in_bytes, f.bd); break;
+    }
+    if (f.bl_on) {
+        int nv = (int)((float)v - f.bl_q[((y & 1) << 1) | (x & 1)]);
+        return iclamp(nv, 0, f.maxv);
+    }
+    if (f.fmt == FMT_U16 && f.bd > 8 && f.bd < 16 && v > f.maxv) v >>= (16 - f.bd);
+    return v;
+}
+
+// Colour role of a site for a pattern: 0 = R, 1 = G, 2 = B
+// (PatternOffsets::from_pattern, include/imop/types.hpp:88-101).
+__host__ __device__ constexpr int r_row_of(int pat) { return (pat == 1 || pat == 3) ? 1 : 0; }
+__host__ __device__ constexpr int r_col_of(int pat) { return (pat == 1 || pat == 2) ? 1 : 0; }
+__host__ __device__ constexpr int site_of(int pat, int y, int x) {
+    return (((y & 1) == r_row_of(pat)) && ((x & 1) == r_col_of(pat))) ? 0
+         : (((y & 1) != r_row_of(pat)) && ((x & 1) != r_col_of(pat))) ? 2 : 1;
+}
+
+// Write one RGB pixel (u8 for bd <= 8, else u16).
+__device__ __forceinline__ void put_px(const FrameParams& f, int x, int y, int r, int g, int b) {
+    const uint64_t i = ((uint64_t)y * (uint64_t)f.w + (uint64_t)x) * 3u;
+    if (f.bd <= 8) {
+        f.out[i] = (uint8_t)r; f.out[i + 1] = (uint8_t)g; f.out[i + 2] = (uint8_t)b;
+    } else {
+        uint16_t* o = (uint16_t*)f.out;
+        o[i] = (uint16_t)r; o[i + 1] = (uint16_t)g; o[i + 2] = (uint16_t)b;
+    }
+}
+
+}  // namespace imop_hip

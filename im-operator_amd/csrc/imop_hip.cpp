@@ -1,0 +1,294 @@
+// imop_hip.cpp -- C ABI of the gfx950 demosaic path (include/imop_hip.h):
+// argument checks with the reference's error codes, frame parameter setup,
+// device workspaces, the host-buffer (synchronous) variants.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "imop_hip.h"
+#include "plain.hpp"
+
+using namespace imop_hip;
+
+namespace {
+
+// imop::DemosaicError (include/imop/types.hpp:31-39)
+enum : int { kOk = 0, kNull = 1, kDims = 2, kBitDepth = 3, kPattern = 4, kSmall = 5, kInternal = 6 };
+// imop::DemosaicAlgorithm (include/imop/types.hpp:19-29)
+enum : int { kSuperFast = 0, kHqli = 1, kMg = 2, kL7 = 3, kDfpd = 4, kAhd = 5, kAmaze = 6, kRcd = 7, kPrism = 8 };
+
+// Minimum frame side per algorithm, as imop::demosaic_cuda checks it
+// (src/algorithms/cuda_dispatch.cpp:77-89).
+int min_dim_of(int algo) {
+    switch (algo) {
+    case kSuperFast: return 2;
+    case kHqli: return 6;
+    case kMg: return 6;
+    case kL7: return 8;
+    case kDfpd: return 12;
+    case kAhd: return 6;
+    case kAmaze: return 6;
+    case kRcd: return 10;
+    case kPrism: return 10;
+    default: return 2;
+    }
+}
+
+uint64_t bayer_bytes(int w, int h, int bd, bool packed) {
+    if (packed) return ((uint64_t)w * (uint64_t)h * (uint64_t)bd + 7u) / 8u;
+    return (uint64_t)w * (uint64_t)h * (bd <= 8 ? 1u : 2u);
+}
+
+uint64_t rgb_bytes(int w, int h, int bd) { return (uint64_t)w * (uint64_t)h * 3u * (bd > 8 ? 2u : 1u); }
+
+int check_args(const void* in, const void* out, int w, int h, int pattern, int algo, int bd, int packed,
+               const float* bl, int bl_mode) {
+    if (!in || !out) return kNull;
+    if (w <= 0 || h <= 0) return kDims;
+    if (bd < 1 || bd > 16) return kBitDepth;
+    if (pattern < 0 || pattern > 3) return kPattern;
+    if (w < min_dim_of(algo) || h < min_dim_of(algo)) return kSmall;
+    if (algo != kHqli && algo != kMg && algo != kPrism) return kInternal;
+    // The reference reads packed streams below 8 bits as one byte per pixel
+    // (pixel::get_raw), past the end of the stream; leave those to it.
+    if (packed && bd < 8) return kInternal;
+    if (bl_mode < 0 || bl_mode > 2 || (bl_mode != 0 && !bl)) return kInternal;
+    return kOk;
+}
+
+// Black-level offsets by CFA parity (y & 1) * 2 + (x & 1), each max(0, .).
+void bl_by_parity(int pattern, const float* offs, int mode, float q[4]) {
+    static const int rr[4] = {0, 1, 0, 1}, rc[4] = {0, 1, 1, 0};
+    for (int py = 0; py < 2; ++py)
+        for (int px = 0; px < 2; ++px) {
+            float o;
+            if (mode == 1) o = offs[0];
+            else {
+                const bool r = py == rr[pattern] && px == rc[pattern];
+                const bool b = py != rr[pattern] && px != rc[pattern];
+                o = r ? offs[0] : b ? offs[3] : (py == rr[pattern] ? offs[1] : offs[2]);
+            }
+            q[py * 2 + px] = o > 0.0f ? o : 0.0f;
+        }
+}
+
+FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int h, int pattern, int bd,
+                        int packed, const float* bl, int bl_mode, int algo, int* mode) {
+    FrameParams f{};
+    f.in = (const uint8_t*)in;
+    f.out = (uint8_t*)out;
+    f.in_bytes = in_bytes;
+    f.w = w;
+    f.h = h;
+    f.bd = bd;
+    f.pattern = pattern;
+    f.maxv = (int)((1u << bd) - 1u);
+    if (bd <= 8) f.fmt = FMT_U8;
+    else if (!packed) f.fmt = FMT_U16;
+    else f.fmt = bd == 12 ? FMT_P12 : bd == 10 ? FMT_P10 : FMT_PGEN;
+    f.bl_on = bl_mode != 0;
+    if (f.bl_on) bl_by_parity(pattern, bl, bl_mode, f.bl_q);
+    // With black level the demosaic runs on the unpacked buffer.
+    const bool eff_packed = packed && !f.bl_on;
+    const bool byte_path = bd <= 8 && !eff_packed;
+    if (algo == kHqli) {
+        *mode = MODE_HQLI;
+        f.clamp_hi = byte_path ? 255 : f.maxv;
+    } else if (algo == kMg) {
+        *mode = byte_path ? MODE_MG_INT8 : MODE_MG_EXACT;
+        f.clamp_hi = byte_path ? 255 : f.maxv;
+    } else {
+        *mode = -1;
+        f.clamp_hi = f.maxv;
+    }
+    return f;
+}
+
+// Per-device scratch for PRISM planes; grows, never shrinks.
+struct DeviceScratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_scratch_mu;
+std::vector<DeviceScratch> g_scratch;
+
+hipError_t get_scratch(size_t need, void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+    DeviceScratch& s = g_scratch[dev];
+    if (s.bytes < need) {
+        if (s.ptr) {
+            // Work already queued may still use the old buffer.
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) return e;
+            (void)hipFree(s.ptr);
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        e = hipMalloc(&s.ptr, need);
+        if (e != hipSuccess) return e;
+        s.bytes = need;
+    }
+    *out = s.ptr;
+    return hipSuccess;
+}
+
+int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int w, int h, int pattern,
+                 int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st) {
+    int e = check_args(d_in, d_out, w, h, pattern, algo, bd, packed, bl, bl_mode);
+    if (e != kOk) return e;
+    if (in_bytes < bayer_bytes(w, h, bd, packed != 0) || out_bytes < rgb_bytes(w, h, bd)) return kInternal;
+    int mode = -1;
+    FrameParams f = make_params(d_in, in_bytes, d_out, w, h, pattern, bd, packed, bl, bl_mode, algo, &mode);
+    hipError_t he;
+    if (algo == kPrism) {
+        void* ws = nullptr;
+        he = get_scratch(prism_workspace_bytes(w, h), &ws);
+        if (he != hipSuccess) return kInternal;
+        he = launch_prism(f, ws, st);
+    } else {
+        he = launch_plain(f, mode, st);
+    }
+    return he == hipSuccess ? kOk : kInternal;
+}
+
+// Per-host-thread device buffers for the host-pointer entry points, like the
+// reference's thread_local CudaWorkspace (cuda_dispatch.cpp:51).
+struct HostStaging {
+    void* d_in = nullptr;
+    size_t in_cap = 0;
+    void* d_out = nullptr;
+    size_t out_cap = 0;
+    int dev = -1;
+    ~HostStaging() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+    }
+    bool ensure(size_t in_b, size_t out_b) {
+        int dev_now = 0;
+        if (hipGetDevice(&dev_now) != hipSuccess) return false;
+        if (dev_now != dev) {
+            if (d_in) (void)hipFree(d_in);
+            if (d_out) (void)hipFree(d_out);
+            d_in = d_out = nullptr;
+            in_cap = out_cap = 0;
+            dev = dev_now;
+        }
+        if (in_cap < in_b) {
+            if (d_in) (void)hipFree(d_in);
+            d_in = nullptr;
+            in_cap = 0;
+            if (hipMalloc(&d_in, in_b) != hipSuccess) return false;
+            in_cap = in_b;
+        }
+        if (out_cap < out_b) {
+            if (d_out) (void)hipFree(d_out);
+            d_out = nullptr;
+            out_cap = 0;
+            if (hipMalloc(&d_out, out_b) != hipSuccess) return false;
+            out_cap = out_b;
+        }
+        return true;
+    }
+};
+thread_local HostStaging t_staging;
+
+int device_count_cached() {
+    static std::once_flag once;
+    static int count = 0;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) == hipSuccess && n > 0) count = n;
+    });
+    return count;
+}
+
+}  // namespace
+
+extern "C" {
+
+int imop_hip_device_count(void) { return device_count_cached(); }
+
+const char* imop_hip_device_name(void) {
+    static char name[256] = "N/A";
+    static std::once_flag once;
+    if (device_count_cached() == 0) return "N/A";
+    std::call_once(once, [] {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, 0) == hipSuccess) {
+            std::strncpy(name, p.name, sizeof(name) - 1);
+            name[sizeof(name) - 1] = '\0';
+        } else {
+            std::strncpy(name, "Unknown", sizeof(name) - 1);
+        }
+    });
+    return name;
+}
+
+int imop_hip_synchronize(void) {
+    if (device_count_cached() == 0) return kInternal;
+    return hipDeviceSynchronize() == hipSuccess ? kOk : kInternal;
+}
+
+int imop_hip_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width, int height,
+                      int pattern, int algorithm, int bit_depth, int is_packed, const float* black_level,
+                      int black_level_mode, void* stream) {
+    if (device_count_cached() == 0) return kInternal;
+    return run_demosaic(d_in, in_bytes, d_out, out_bytes, width, height, pattern, algorithm, bit_depth, is_packed,
+                        black_level, black_level_mode, (hipStream_t)stream);
+}
+
+int imop_hip_demosaic_host(const uint8_t* bayer, uint8_t* rgb, int width, int height, int pattern, int algorithm,
+                           int bit_depth, int is_packed, const float* black_level, int black_level_mode) {
+    if (device_count_cached() == 0) return kInternal;
+    int e = check_args(bayer, rgb, width, height, pattern, algorithm, bit_depth, is_packed, black_level,
+                       black_level_mode);
+    if (e != kOk) return e;
+    const uint64_t ib = bayer_bytes(width, height, bit_depth, is_packed != 0);
+    const uint64_t ob = rgb_bytes(width, height, bit_depth);
+    if (!t_staging.ensure(ib, ob)) return kInternal;
+    if (hipMemcpy(t_staging.d_in, bayer, ib, hipMemcpyHostToDevice) != hipSuccess) return kInternal;
+    e = run_demosaic(t_staging.d_in, ib, t_staging.d_out, ob, width, height, pattern, algorithm, bit_depth,
+                     is_packed, black_level, black_level_mode, nullptr);
+    if (e != kOk) return e;
+    if (hipMemcpy(rgb, t_staging.d_out, ob, hipMemcpyDeviceToHost) != hipSuccess) return kInternal;
+    return kOk;
+}
+
+int imop_hip_black_level(void* d_data, int width, int height, int pattern, int algorithm, int bit_depth,
+                         const float* offsets, void* stream) {
+    // black_level::BlackLevelError: NullInput 1, InvalidDimensions 2,
+    // InvalidBitDepth 3, InternalError 6 (include/black_level/types.hpp).
+    if (device_count_cached() == 0) return kInternal;
+    if (!d_data || !offsets) return kNull;
+    if (width <= 0 || height <= 0) return kDims;
+    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    if (pattern < 0 || pattern > 3 || algorithm < 0 || algorithm > 1) return kInternal;
+    float q[4];
+    bl_by_parity(pattern, offsets, algorithm == 1 ? 1 : 2, q);
+    return launch_black_level((uint8_t*)d_data, width, height, bit_depth, q, (hipStream_t)stream) == hipSuccess
+               ? kOk
+               : kInternal;
+}
+
+int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern, int algorithm, int bit_depth,
+                              const float* offsets) {
+    if (device_count_cached() == 0) return kInternal;
+    if (!data || !offsets) return kNull;
+    if (width <= 0 || height <= 0) return kDims;
+    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    const uint64_t nb = bayer_bytes(width, height, bit_depth, false);
+    if (!t_staging.ensure(nb, 1)) return kInternal;
+    if (hipMemcpy(t_staging.d_in, data, nb, hipMemcpyHostToDevice) != hipSuccess) return kInternal;
+    int e = imop_hip_black_level(t_staging.d_in, width, height, pattern, algorithm, bit_depth, offsets, nullptr);
+    if (e != kOk) return e;
+    if (hipMemcpy(data, t_staging.d_in, nb, hipMemcpyDeviceToHost) != hipSuccess) return kInternal;
+    return kOk;
+}
+
+}  // extern "C"

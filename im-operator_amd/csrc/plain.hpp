@@ -1,0 +1,19 @@
+// plain.hpp -- launchers shared between the kernel files and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "device_common.hpp"
+
+namespace imop_hip {
+
+// HQLI / MG tiled kernel for one frame (plain_kernels.hip).
+hipError_t launch_plain(const FrameParams& f, int mode, hipStream_t st);
+
+// In-place black level on an unpacked frame; q = offsets by (y & 1) * 2 + (x & 1).
+hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q[4], hipStream_t st);
+
+// PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
+size_t prism_workspace_bytes(int w, int h);
+hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st);
+
+}  // namespace imop_hip

@@ -1,0 +1,180 @@
+"""imop_amd -- Python binding of the gfx950 Bayer demosaic library.
+
+The product is libimop_hip.so (C ABI in include/imop_hip.h).  This package
+loads it with ctypes and exposes the reference operator interface for the
+demosaic path:
+
+  * demosaic(...)          imop::demosaic_cuda with device tensors (stream-ordered)
+  * demosaic_host(...)     imop::demosaic_cuda with host arrays (synchronous)
+  * black_level(...)       black_level::process_black_level_cuda
+
+Names, enumerations and error codes follow the reference
+(include/imop/types.hpp:12-39, include/imop/algorithms.hpp:19-84).  If the
+library is missing every call raises: there is no CPU fallback in this
+package.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libimop_hip.so")
+
+
+class BayerPattern(enum.IntEnum):
+    """include/imop/types.hpp:12-17"""
+    RGGB = 0
+    BGGR = 1
+    GRBG = 2
+    GBRG = 3
+
+
+class DemosaicAlgorithm(enum.IntEnum):
+    """include/imop/types.hpp:19-29"""
+    SUPER_FAST = 0
+    HQLI = 1
+    MG = 2
+    L7 = 3
+    DFPD = 4
+    AHD = 5
+    AMAZE = 6
+    RCD = 7
+    PRISM = 8
+
+
+class DemosaicError(enum.IntEnum):
+    """include/imop/types.hpp:31-39"""
+    Ok = 0
+    NullInput = 1
+    InvalidDimensions = 2
+    InvalidBitDepth = 3
+    InvalidPattern = 4
+    ImageTooSmall = 5
+    InternalError = 6
+
+
+class BlackLevelAlgorithm(enum.IntEnum):
+    """include/black_level/types.hpp"""
+    PER_CHANNEL = 0
+    GLOBAL = 1
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libimop_hip.so; raise LibraryMissing when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} not built (run __graft_entry__.build() or make -C im-operator_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u8p, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)
+        i, u64 = ctypes.c_int, ctypes.c_uint64
+        L.imop_hip_device_count.restype = i
+        L.imop_hip_device_name.restype = ctypes.c_char_p
+        L.imop_hip_synchronize.restype = i
+        L.imop_hip_demosaic.argtypes = [vp, u64, vp, u64, i, i, i, i, i, i, fp, i, vp]
+        L.imop_hip_demosaic.restype = i
+        L.imop_hip_demosaic_host.argtypes = [u8p, u8p, i, i, i, i, i, i, fp, i]
+        L.imop_hip_demosaic_host.restype = i
+        L.imop_hip_black_level.argtypes = [vp, i, i, i, i, i, fp, vp]
+        L.imop_hip_black_level.restype = i
+        L.imop_hip_black_level_host.argtypes = [u8p, i, i, i, i, i, fp]
+        L.imop_hip_black_level_host.restype = i
+        _lib = L
+    return _lib
+
+
+def has_cuda() -> bool:
+    """imop::has_cuda (src/algorithms/cuda_dispatch.cpp:26-33)."""
+    return lib().imop_hip_device_count() > 0
+
+
+def cuda_device_name() -> str:
+    return lib().imop_hip_device_name().decode()
+
+
+def cuda_synchronize() -> None:
+    lib().imop_hip_synchronize()
+
+
+def bayer_byte_size(w: int, h: int, bd: int, packed: bool) -> int:
+    """pixel::compute_bayer_byte_size (include/imop/pixel_utils.hpp:25-28)."""
+    if packed:
+        return (w * h * bd + 7) // 8
+    return w * h * (1 if bd <= 8 else 2)
+
+
+def rgb_byte_size(w: int, h: int, bd: int) -> int:
+    """pixel::compute_rgb_byte_size (include/imop/pixel_utils.hpp:30-34)."""
+    return w * h * 3 * (2 if bd > 8 else 1)
+
+
+def _bl_args(black_level, per_channel: bool):
+    if black_level is None:
+        return None, 0
+    arr = np.asarray(black_level, dtype=np.float32).ravel()
+    if arr.size == 1:
+        arr = np.repeat(arr, 4)
+    arr = np.ascontiguousarray(arr[:4], dtype=np.float32)
+    return arr, (2 if per_channel else 1)
+
+
+def demosaic(d_in, d_out, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,
+             is_packed: bool = False, black_level=None, per_channel: bool = False, stream=None) -> int:
+    """Stream-ordered demosaic of device tensors (torch, any dtype, contiguous).
+
+    Returns the DemosaicError value; kernels are enqueued on `stream`
+    (a torch.cuda.Stream, a raw hipStream_t int, or None for the current
+    torch stream)."""
+    bl, mode = _bl_args(black_level, per_channel)
+    blp = bl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if bl is not None else None
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream().cuda_stream
+    elif hasattr(stream, "cuda_stream"):
+        stream = stream.cuda_stream
+    in_bytes = d_in.numel() * d_in.element_size()
+    out_bytes = d_out.numel() * d_out.element_size()
+    return lib().imop_hip_demosaic(ctypes.c_void_p(d_in.data_ptr()), in_bytes, ctypes.c_void_p(d_out.data_ptr()),
+                                   out_bytes, width, height, int(pattern), int(algorithm), bit_depth,
+                                   int(bool(is_packed)), blp, mode, ctypes.c_void_p(stream))
+
+
+def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,
+                  is_packed: bool = False, black_level=None, per_channel: bool = False,
+                  out: np.ndarray | None = None):
+    """imop::demosaic_cuda on host buffers: returns (DemosaicError, rgb)."""
+    raw = np.ascontiguousarray(bayer).view(np.uint8).ravel()
+    if out is None:
+        out = np.zeros(rgb_byte_size(width, height, bit_depth), dtype=np.uint8)
+    bl, mode = _bl_args(black_level, per_channel)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    blp = bl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if bl is not None else None
+    e = lib().imop_hip_demosaic_host(raw.ctypes.data_as(u8p), out.ctypes.data_as(u8p), width, height, int(pattern),
+                                     int(algorithm), bit_depth, int(bool(is_packed)), blp, mode)
+    rgb = out.view(np.uint16) if bit_depth > 8 else out
+    return DemosaicError(e), rgb.reshape(height, width, 3)
+
+
+def black_level(d_data, width: int, height: int, pattern: int, algorithm: int, bit_depth: int, offsets,
+                stream=None) -> int:
+    """In-place black level of a device tensor (black_level::process_black_level_cuda)."""
+    o = np.ascontiguousarray(np.asarray(offsets, dtype=np.float32).ravel()[:4])
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream().cuda_stream
+    elif hasattr(stream, "cuda_stream"):
+        stream = stream.cuda_stream
+    return lib().imop_hip_black_level(ctypes.c_void_p(d_data.data_ptr()), width, height, int(pattern),
+                                      int(algorithm), bit_depth, o.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                      ctypes.c_void_p(stream))

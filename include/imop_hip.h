@@ -1,0 +1,93 @@
+/*
+ * imop_hip.h -- C ABI of the MI355X (gfx950) Bayer demosaic path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int holding
+ * an imop::DemosaicError value (include/imop/types.hpp:31-39 of the
+ * reference): 0 = Ok, 1 = NullInput, 2 = InvalidDimensions,
+ * 3 = InvalidBitDepth, 4 = InvalidPattern, 5 = ImageTooSmall,
+ * 6 = InternalError.  As in the reference, any non-zero result means "no
+ * output was produced"; imop::demosaic() then falls back to its CPU path
+ * (src/algorithms/dispatch.cpp:96-111).
+ *
+ * Enumerations are the reference's: pattern = imop::BayerPattern
+ * (RGGB 0, BGGR 1, GRBG 2, GBRG 3), algorithm = imop::DemosaicAlgorithm
+ * (HQLI 1, MG 2, PRISM 8 have kernels; the other registry entries return 6).
+ *
+ * Raw input layouts (include/imop/pixel_utils.hpp:21-189): bit_depth <= 8 one
+ * byte per pixel; bit_depth > 8 unpacked is little-endian u16 per pixel;
+ * packed 12-bit is 2 px / 3 B, packed 10-bit 4 px / 5 B, other packed depths
+ * an LSB-first bitstream, all over the linear pixel index with no row
+ * padding.  Output is interleaved RGB, u8 for bit_depth <= 8, else u16.
+ *
+ * Black level (black_level/global.cpp:6-24, per_channel.cpp:6-33):
+ * black_level_mode 0 = none; 1 = global (offsets[0] for every site);
+ * 2 = per channel (offsets = R, Gr, Gb, B).  When enabled the result equals
+ * unpack -> process_black_level -> demosaic of the unpacked buffer.
+ *
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef IMOP_HIP_H
+#define IMOP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMOP_HIP_ABI_VERSION 1
+
+/* Replaces imop::has_cuda (src/algorithms/cuda_dispatch.cpp:26-33):
+ * number of visible HIP devices, 0 when none. */
+int imop_hip_device_count(void);
+
+/* Replaces imop::cuda_device_name (src/algorithms/cuda_dispatch.cpp:35-49):
+ * name of device 0, "N/A" without a device. */
+const char* imop_hip_device_name(void);
+
+/* Replaces imop::cuda_synchronize (src/algorithms/cuda_dispatch.cpp:211-213). */
+int imop_hip_synchronize(void);
+
+/* Stream-ordered demosaic of one frame resident in device memory.
+ * Replaces the kernel launches behind imop::demosaic_cuda
+ * (src/algorithms/cuda_dispatch.cpp:102-135 -> cuda_launch_hqli / _mg /
+ * _prism, src/algorithms/cuda_kernels.cu:1690-1866), fused with unpack and
+ * black level.  d_in holds in_bytes bytes, d_out out_bytes bytes; both must
+ * be at least the sizes pixel::compute_bayer_byte_size /
+ * compute_rgb_byte_size give.  black_level points to 4 host floats (may be
+ * NULL when black_level_mode is 0).  Returns after enqueueing. */
+int imop_hip_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
+                      int height, int pattern, int algorithm, int bit_depth, int is_packed,
+                      const float* black_level, int black_level_mode, void* stream);
+
+/* imop::demosaic_cuda (src/algorithms/cuda_dispatch.cpp:68-141) with the same
+ * host-buffer, synchronous contract, plus packed input and black level.
+ * Buffers are caller-owned host memory sized by compute_bayer_byte_size /
+ * compute_rgb_byte_size; the result is valid on return. */
+int imop_hip_demosaic_host(const uint8_t* bayer, uint8_t* rgb, int width, int height, int pattern,
+                           int algorithm, int bit_depth, int is_packed, const float* black_level,
+                           int black_level_mode);
+
+/* Black level in place on an unpacked Bayer frame in device memory
+ * (u8 for bit_depth <= 8, else u16).  algorithm = black_level::
+ * BlackLevelAlgorithm (PER_CHANNEL 0, GLOBAL 1); offsets = R, Gr, Gb, B
+ * (host floats).  Replaces the kernels behind
+ * black_level::process_black_level_cuda (src/gpu_dispatch.cpp:421-448 ->
+ * src/gpu_ops_kernels.cu:592-627) with the CPU module's semantics
+ * (src/black_level/global.cpp:6-24, per_channel.cpp:6-33).  Returns a
+ * black_level::BlackLevelError value (same numbering, 4 = InvalidChannels
+ * unused). */
+int imop_hip_black_level(void* d_data, int width, int height, int pattern, int algorithm, int bit_depth,
+                         const float* offsets, void* stream);
+
+/* black_level::process_black_level_cuda (src/gpu_dispatch.cpp:421-448):
+ * host buffer, synchronous. */
+int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern, int algorithm, int bit_depth,
+                              const float* offsets);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMOP_HIP_H */

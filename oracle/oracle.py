@@ -39,15 +39,6 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_u8p)
 
 
-class _Lib:
-    def __init__(self, path: str, prefix: str):
-        self.lib = ctypes.CDLL(path)
-        self.prefix = prefix
-
-    def fn(self, name):
-        return getattr(self.lib, self.prefix + name)
-
-
 _oracle = None
 _ref = None
 
@@ -86,6 +77,7 @@ def reference() -> ctypes.CDLL:
         lib.ref_get_raw.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_size_t, ctypes.c_int]
         lib.ref_set_num_threads.argtypes = [ctypes.c_int]
+        lib.ref_unpack.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_u8p]
         _ref = lib
     return _ref
 
@@ -138,11 +130,8 @@ def ref_demosaic(raw: np.ndarray, w: int, h: int, pattern: int, algo: int, bd: i
             bl = np.repeat(bl.reshape(1), 4)
         bl = np.ascontiguousarray(bl, dtype=np.float32)
         if packed:
-            n = w * h
-            un = np.zeros(n * (2 if bd > 8 else 1), dtype=np.uint8)
-            vals = np.array([lib.ref_get_raw(_ptr(raw), i % w, i // w, w, bd, 1, 0, h)
-                             for i in range(n)], dtype=np.uint16 if bd > 8 else np.uint8)
-            un[:] = vals.view(np.uint8)
+            un = np.zeros(w * h * (2 if bd > 8 else 1), dtype=np.uint8)
+            lib.ref_unpack(_ptr(raw), w, h, bd, 1, _ptr(un))
             raw = un
         e = lib.ref_black_level(_ptr(raw), w, h, pattern, bd, bl.ctypes.data_as(_fp),
                                 int(per_channel))

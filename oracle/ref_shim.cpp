@@ -57,6 +57,17 @@ int ref_get_raw(const uint8_t* data, int x, int y, int w, int bd, int packed, si
     return imop::pixel::get_raw(data, x, y, w, bd, packed != 0, nbytes, h);
 }
 
+// The caller-side unpack that precedes black level: get_raw for every pixel
+// into a u8 (bd <= 8) or u16 buffer.
+void ref_unpack(const uint8_t* data, int w, int h, int bd, int packed, uint8_t* out) {
+    const size_t n = static_cast<size_t>(w) * h;
+    for (size_t i = 0; i < n; ++i) {
+        int v = imop::pixel::get_raw(data, static_cast<int>(i % w), static_cast<int>(i / w), w, bd, packed != 0, 0, h);
+        if (bd <= 8) out[i] = static_cast<uint8_t>(v);
+        else imop::pixel::write_u16(out, 2 * i, static_cast<uint16_t>(v));
+    }
+}
+
 int ref_has_avx2() { return imop::has_avx2() ? 1 : 0; }
 
 int ref_num_threads() {
