@@ -40,6 +40,19 @@ struct FrameParams {
     float bl_q[4];        // offset by (y & 1) * 2 + (x & 1), already max(0, .)
 };
 
+// Parameters of the register-resident HQLI / MG kernel (fast_kernels.hip).
+struct FastParams {
+    const uint8_t* in;
+    uint8_t* out;
+    uint64_t in_bytes;
+    int32_t w, h, bd, maxv, clamp_hi, pattern;
+    int32_t bl_on;
+    int32_t bl_i[4];      // integral black level by (y & 1) * 2 + (x & 1)
+    int32_t th;           // rows per wave (even)
+    int32_t nsx, nby;     // strips (248 columns each) and bands
+    int32_t nwaves;
+};
+
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return imin(imax(v, lo), hi); }

@@ -1,0 +1,432 @@
+// fast_impl.hpp -- register-resident fused unpack -> black level ->
+// HQLI / MG demosaic for gfx950.
+//
+// Geometry.  A wave owns a strip of 64 lanes x 4 pixels = 256 columns and a
+// band of TH rows.  Lanes 1..62 produce output (248 columns); lanes 0 and 63
+// only load, so that every producing lane finds its horizontal neighbours in
+// the adjacent lanes (DPP wave_shr / wave_shl, one VALU op per value).  Strips
+// overlap by 8 columns; consecutive strips' producing lanes tile a row.
+//
+// Each lane walks down its band two rows (one CFA quad row) per step, keeping
+// a sliding window of 6 raw rows and 4 rows of MG colour differences in
+// registers, so every packed byte is loaded once per band (plus a 6-row
+// apron) with one aligned 4- or 8-byte load per lane and row, and every
+// output row is written once with 8-byte stores, coalesced across the wave.
+// No LDS and no barriers.
+//
+// Arithmetic is the exact integer form of the reference algorithms (see
+// plain_kernels.hip): MG in units of 1/32 (float path) or the reference's own
+// 8-bit integer rounding; HQLI integer averages.  The border rules of
+// fill_intermediate_borders / fill_rgb_borders (common.hpp:47-129) are applied
+// in the waves that touch the frame edge ("careful" bands): the pre-fill value
+// of every position is computed, then rows 0-1 / h-2..h-1 copy rows 2 / h-3
+// as they are before the column fill, and columns 0-1 / w-2..w-1 copy
+// columns 2 / w-3.
+//
+// Preconditions (checked on the host, else the generic LDS path runs):
+// width % 4 == 0, layout u8 / u16 / 12-bit packed / 10-bit packed, black
+// level absent or integral per site.
+#pragma once
+
+#include "device_common.hpp"
+#include "plain.hpp"
+
+namespace imop_hip {
+
+namespace {
+
+// (lanes without a neighbour read 0; only halo lanes see that)
+__device__ __forceinline__ int from_left(int v) {   // lane i <- lane i-1
+    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int from_right(int v) {  // lane i <- lane i+1
+    return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
+}
+
+// clamp(v, 0, hi); with a constant hi the compiler forms one v_med3_i32
+__device__ __forceinline__ int clamp0(int v, int hi) { return imin(imax(v, 0), hi); }
+
+__device__ __forceinline__ int tdiv4(int v) { return (v + ((v >> 31) & 3)) >> 2; }           // C's v / 4
+__device__ __forceinline__ int tdiv2(int v) { return (v + (int)((unsigned)v >> 31)) >> 1; }  // C's v / 2
+
+// Raw bytes of one row chunk of a lane: 4 samples at x (x % 4 == 0).  All
+// loads of a band are issued back to back with no control flow between
+// them: rows outside the frame read offset 0 and are zeroed when decoded.
+struct Raw {
+    uint32_t lo, hi;
+};
+
+// Byte offset of the group holding sample (x, y) = row part + lane part
+// (x % 4 == 0 and w % 4 == 0 keep both parts whole numbers of bytes).
+template <int FMT>
+__device__ __forceinline__ uint32_t row_stride(uint32_t w) {
+    return FMT == FMT_U8 ? w : FMT == FMT_U16 ? 2u * w : FMT == FMT_P12 ? w / 2u * 3u : w / 4u * 5u;
+}
+template <int FMT>
+__device__ __forceinline__ uint32_t lane_offset(uint32_t x) {
+    return FMT == FMT_U8 ? x : FMT == FMT_U16 ? 2u * x : FMT == FMT_P12 ? x / 2u * 3u : x / 4u * 5u;
+}
+
+template <int FMT>
+__device__ __forceinline__ Raw load_raw(const FastParams& f, uint32_t off, bool ok) {
+    Raw r;
+    if (FMT == FMT_U8) {
+        r.lo = *(const uint32_t*)(f.in + (ok ? off : 0u));
+        r.hi = 0;
+    } else if (FMT == FMT_U16) {
+        uint2 v = *(const uint2*)(f.in + (ok ? off : 0u));
+        r.lo = v.x; r.hi = v.y;
+    } else {
+        uint2 v = *(const uint2*)(f.in + (ok ? (off & ~3u) : 0u));
+        const uint32_t s = off & 3u;
+        r.lo = __builtin_amdgcn_alignbyte(v.y, v.x, s);  // bytes 0..3 of the group run
+        r.hi = v.y >> (8u * s);                           // bytes 4..
+    }
+    return r;
+}
+
+// Decode a raw chunk to 4 samples; black level (integral offsets o0 / o1 for
+// even / odd columns of the row) or MSB alignment applied.
+template <int FMT, bool MASK>
+__device__ __forceinline__ void decode4(const FastParams& f, Raw r, bool ok, int o0, int o1, int p[4]) {
+    uint32_t v[4];
+    if (FMT == FMT_U8) {
+        v[0] = r.lo & 0xFF; v[1] = (r.lo >> 8) & 0xFF; v[2] = (r.lo >> 16) & 0xFF; v[3] = r.lo >> 24;
+    } else if (FMT == FMT_U16) {
+        v[0] = r.lo & 0xFFFF; v[1] = r.lo >> 16; v[2] = r.hi & 0xFFFF; v[3] = r.hi >> 16;
+    } else if (FMT == FMT_P12) {
+        const uint32_t b1 = (r.lo >> 8) & 0xFF, b4 = r.hi & 0xFF;
+        v[0] = ((r.lo & 0xFF) << 4) | (b1 & 0xF);
+        v[1] = (((r.lo >> 16) & 0xFF) << 4) | (b1 >> 4);
+        v[2] = ((r.lo >> 24) << 4) | (b4 & 0xF);
+        v[3] = (((r.hi >> 8) & 0xFF) << 4) | (b4 >> 4);
+    } else {
+        const uint32_t t = r.hi & 0xFF;
+        v[0] = ((r.lo & 0xFF) << 2) | (t & 3);
+        v[1] = (((r.lo >> 8) & 0xFF) << 2) | ((t >> 2) & 3);
+        v[2] = (((r.lo >> 16) & 0xFF) << 2) | ((t >> 4) & 3);
+        v[3] = ((r.lo >> 24) << 2) | (t >> 6);
+    }
+    if (FMT != FMT_U16) {
+        // decoded samples never exceed max; offsets are 0 without black level
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = __builtin_elementwise_sub_sat(v[k], (uint32_t)((k & 1) ? o1 : o0));
+    } else if (f.bl_on) {
+        // raw u16 may exceed max: black level clamps both ways
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = (uint32_t)imin((int)__builtin_elementwise_sub_sat(v[k], (uint32_t)((k & 1) ? o1 : o0)), f.maxv);
+    } else if (f.bd < 16) {
+        const int sh = 16 - f.bd;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (int)v[k] > f.maxv ? v[k] >> sh : v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = (!MASK || ok) ? (int)v[k] : 0;
+}
+
+// The stream's final row of a packed layout: per-sample decode that keeps
+// get_packed_raw's bound (a group touching the last byte reads 0).
+template <int FMT>
+__device__ __forceinline__ void decode_last_row(const FastParams& f, uint32_t y, uint32_t x, bool ok, int o0, int o1,
+                                                int p[4]) {
+    const uint64_t idx = (uint64_t)y * f.w + x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = 0;
+        if (ok) v = (FMT == FMT_P12) ? decode_p12(f.in, idx + k, f.in_bytes) : decode_p10(f.in, idx + k, f.in_bytes);
+        if (f.bl_on) v = imax(v - ((k & 1) ? o1 : o0), 0);
+        p[k] = ok ? v : 0;
+    }
+}
+
+// Colour role of CFA position parity (py, px) for the pattern.
+template <int PAT>
+__device__ constexpr int role(int py, int px) { return site_of(PAT, py, px); }
+
+// Column parity of the R/B sites in a row of parity py.
+template <int PAT>
+__device__ constexpr int rb_col_parity(int py) { return py == r_row_of(PAT) ? r_col_of(PAT) : 1 - r_col_of(PAT); }
+
+// Store 4 RGB pixels of one row at (x, y) (x % 4 == 0); 32-bit offsets.
+__device__ __forceinline__ void store4(const FastParams& f, uint32_t x, uint32_t y, const int r[4], const int g[4],
+                                       const int b[4]) {
+    const uint32_t i = (y * (uint32_t)f.w + x) * 3u;
+    if (f.bd > 8) {
+        uint2* o = (uint2*)(f.out + 2u * i);
+        o[0] = make_uint2((uint32_t)r[0] | ((uint32_t)g[0] << 16), (uint32_t)b[0] | ((uint32_t)r[1] << 16));
+        o[1] = make_uint2((uint32_t)g[1] | ((uint32_t)b[1] << 16), (uint32_t)r[2] | ((uint32_t)g[2] << 16));
+        o[2] = make_uint2((uint32_t)b[2] | ((uint32_t)r[3] << 16), (uint32_t)g[3] | ((uint32_t)b[3] << 16));
+    } else {
+        uint32_t* o = (uint32_t*)(f.out + i);
+        o[0] = (uint32_t)r[0] | ((uint32_t)g[0] << 8) | ((uint32_t)b[0] << 16) | ((uint32_t)r[1] << 24);
+        o[1] = (uint32_t)g[1] | ((uint32_t)b[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)g[2] << 24);
+        o[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
+    }
+}
+
+// MG colour difference of the two R/B sites a lane owns in one row.
+// P points at 5 window rows centred on the row (P[0] two rows up).
+// Halo columns come from the neighbouring lanes.
+template <int PAT, int MODE, bool CAREFUL>
+__device__ __forceinline__ void d_row(const int (*P)[4], int py, int row_ok, const bool col_ok[4], int D[3]) {
+    const int pc = rb_col_parity<PAT>(py);
+    const int* c = P[2];
+    // row values at lane columns -2..5
+    int v[8];
+    v[0] = from_left(c[2]);
+    v[1] = from_left(c[3]);
+    v[2] = c[0]; v[3] = c[1]; v[4] = c[2]; v[5] = c[3];
+    v[6] = from_right(c[0]);
+    v[7] = from_right(c[1]);
+    int own[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int cj = pc + 2 * j;  // lane column of the site
+        const int cc = v[cj + 2];
+        const int nr = P[1][cj] + P[3][cj] + v[cj + 1] + v[cj + 3];
+        const int fr = P[0][cj] + P[4][cj] + v[cj] + v[cj + 4];
+        const int gn = 4 * cc + 2 * nr - fr;
+        int d;
+        if (MODE == MODE_MG_EXACT) d = 8 * cc - gn;
+        else d = cc - clamp0((gn + 4) >> 3, 255);
+        if (CAREFUL) {
+            const int sentinel = (MODE == MODE_MG_EXACT) ? 8 * cc + 8 : cc;
+            d = (row_ok && col_ok[cj]) ? d : sentinel;
+        }
+        own[j] = d;
+    }
+    if (pc == 0) { D[0] = own[0]; D[1] = own[1]; D[2] = from_right(own[0]); }
+    else { D[0] = from_left(own[1]); D[1] = own[0]; D[2] = own[1]; }
+}
+
+// D value of the R/B site at lane column cr (-1..4) of a D row of parity py.
+template <int PAT>
+__device__ __forceinline__ int dat(const int D[3], int py, int cr) {
+    return D[(cr + rb_col_parity<PAT>(py)) >> 1];
+}
+
+// One output pixel of the step from the window: row a (0/1) of the quad row,
+// lane column k (0..3).  D rows: Dm = row above, D0 = own row, Dp = row below.
+template <int PAT, int MODE>
+__device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const int* D0, const int* Dp, int ch,
+                                      int& r, int& g, int& b) {
+    const int s = role<PAT>(a, k & 1);
+    const int pm = (a + 1) & 1, p0 = a & 1, pp = (a + 1) & 1;
+    if (s != 1) {
+        const int dc = dat<PAT>(D0, p0, k);
+        const int ds = dat<PAT>(Dm, pm, k - 1) + dat<PAT>(Dm, pm, k + 1) + dat<PAT>(Dp, pp, k - 1) +
+                       dat<PAT>(Dp, pp, k + 1);
+        int gg, oth;
+        if (MODE == MODE_MG_EXACT) {
+            const int t = 8 * c - dc + 4;           // 8 * G + 4
+            gg = t >> 3;
+            oth = (4 * t + ds) >> 5;                 // (32c - 4D + sum D + 16) >> 5
+        } else {
+            gg = c - dc;
+            oth = gg + tdiv4(ds + 2);
+        }
+        gg = clamp0(gg, ch);
+        oth = clamp0(oth, ch);
+        if (s == 0) { r = c; g = gg; b = oth; } else { b = c; g = gg; r = oth; }
+    } else {
+        const int hs = dat<PAT>(D0, p0, k - 1) + dat<PAT>(D0, p0, k + 1);
+        const int vs = dat<PAT>(Dm, pm, k) + dat<PAT>(Dp, pp, k);
+        int hz, vt;
+        if (MODE == MODE_MG_EXACT) {
+            const int u = 16 * c + 8;
+            hz = (u + hs) >> 4;
+            vt = (u + vs) >> 4;
+        } else {
+            hz = c + tdiv2(hs + 1);
+            vt = c + tdiv2(vs + 1);
+        }
+        hz = clamp0(hz, ch);
+        vt = clamp0(vt, ch);
+        g = c;
+        if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
+    }
+}
+
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH>
+__device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, bool lane_out) {
+    constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
+    const int w = f.w, h = f.h;
+    const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
+    const bool lane_ok = x >= 0 && x < w;
+    const uint32_t xu = lane_ok ? (uint32_t)x : 0u;
+    bool col_ok[4], col_core[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        col_ok[k] = (x + k >= 2) && (x + k < w - 2);     // interior site column (MG green kernel)
+        col_core[k] = (x + k >= 1) && (x + k < w - 1);   // computed by the colour-difference pass
+    }
+    constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10;
+
+    // issue every load of the band (interior bands: every row and lane is
+    // inside the frame, so no masking)
+    const uint32_t rs = row_stride<FMT>((uint32_t)w), lo = lane_offset<FMT>(xu);
+    Raw raw[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int y = y0 - 3 + i;
+        const bool row_ok = !CAREFUL || (y >= 0 && y < h && !(packed && y == h - 1));
+        raw[i] = load_raw<FMT>(f, (row_ok ? (uint32_t)y : 0u) * rs + lo, !CAREFUL || (row_ok && lane_ok));
+    }
+
+    int P[NR][4];
+    auto decode_row = [&](int i) {
+        const int y = y0 - 3 + i;
+        const int o0 = f.bl_i[(y & 1) * 2], o1 = f.bl_i[(y & 1) * 2 + 1];
+        const bool row_ok = y >= 0 && y < h;
+        // only careful bands reach the stream's final row
+        if (CAREFUL && packed && y == h - 1) decode_last_row<FMT>(f, (uint32_t)y, xu, lane_ok, o0, o1, P[i]);
+        else decode4<FMT, CAREFUL>(f, raw[i], row_ok && lane_ok, o0, o1, P[i]);
+    };
+
+    int D[TH + 2][3];  // MG colour differences, rows y0-1 .. y0+TH
+    auto d_row_at = [&](int j) {  // D row j <-> image row y0 - 1 + j, window centre P[j + 2]
+        const int y = y0 - 1 + j;
+        d_row<PAT, MODE, CAREFUL>(&P[j], (j + 1) & 1, (y >= 2) && (y < h - 2), col_ok, D[j]);
+    };
+
+#pragma unroll
+    for (int i = 0; i < 6; ++i) decode_row(i);
+    if (MODE != MODE_HQLI) { d_row_at(0); d_row_at(1); }
+
+#pragma unroll
+    for (int st = 0; st < TH / 2; ++st) {
+        const int yA = y0 + 2 * st;
+        decode_row(2 * st + 6);
+        decode_row(2 * st + 7);
+        if (MODE != MODE_HQLI) { d_row_at(2 * st + 2); d_row_at(2 * st + 3); }
+        int R[2][4], G[2][4], B[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int* p = P[2 * st + 3 + a];
+            const int* pu = P[2 * st + 2 + a];
+            const int* pd = P[2 * st + 4 + a];
+            int hl = 0, hr = 0, ul = 0, ur = 0, dl = 0, dr = 0;
+            if (MODE == MODE_HQLI) {
+                hl = from_left(p[3]); hr = from_right(p[0]);
+                ul = from_left(pu[3]); ur = from_right(pu[0]);
+                dl = from_left(pd[3]); dr = from_right(pd[0]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = p[k];
+                int r, g, b;
+                if (MODE == MODE_HQLI) {
+                    const int s = role<PAT>(a, k & 1);
+                    const int le = k > 0 ? p[k - 1] : hl, ri = k < 3 ? p[k + 1] : hr;
+                    const int up = pu[k], dn = pd[k];
+                    if (s != 1) {
+                        const int ul_ = k > 0 ? pu[k - 1] : ul, ur_ = k < 3 ? pu[k + 1] : ur;
+                        const int dl_ = k > 0 ? pd[k - 1] : dl, dr_ = k < 3 ? pd[k + 1] : dr;
+                        const int gg = (le + ri + up + dn + 2) >> 2;
+                        const int oth = (ul_ + ur_ + dl_ + dr_ + 2) >> 2;
+                        if (s == 0) { r = c; g = gg; b = oth; } else { b = c; g = gg; r = oth; }
+                    } else {
+                        const int hz = (le + ri + 1) >> 1, vt = (up + dn + 1) >> 1;
+                        g = c;
+                        if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
+                    }
+                    r = imin(r, ch); g = imin(g, ch); b = imin(b, ch);
+                } else {
+                    mg_px<PAT, MODE>(a, k, c, D[2 * st + a], D[2 * st + a + 1], D[2 * st + a + 2], ch, r, g, b);
+                    if (CAREFUL) {
+                        // positions the colour-difference pass never reaches keep R = B = 0
+                        const bool core = col_core[k] && (yA + a >= 1) && (yA + a < h - 1);
+                        r = core ? r : 0;
+                        b = core ? b : 0;
+                    }
+                }
+                R[a][k] = r; G[a][k] = g; B[a][k] = b;
+            }
+        }
+
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int y = yA + a;
+            if (!CAREFUL) {
+                if (lane_out) store4(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
+                continue;
+            }
+            if (y >= h || y < 2 || y >= h - 2 || !lane_out) continue;
+            int r[4], g[4], b[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { r[k] = R[a][k]; g[k] = G[a][k]; b[k] = B[a][k]; }
+            if (MODE == MODE_HQLI) {
+                // pre-fill value of a border column: never written -> 0
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (!col_ok[k]) { r[k] = 0; g[k] = 0; b[k] = 0; }
+            }
+            if (y == 2) { store4(f, x, 0, r, g, b); store4(f, x, 1, r, g, b); }
+            if (y == h - 3) { store4(f, x, h - 2, r, g, b); store4(f, x, h - 1, r, g, b); }
+            // column fill of the middle rows: columns 0-1 <- 2, w-2..w-1 <- w-3
+            if (x == 0) { r[0] = r[1] = R[a][2]; g[0] = g[1] = G[a][2]; b[0] = b[1] = B[a][2]; }
+            if (x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
+            store4(f, x, y, r, g, b);
+        }
+    }
+}
+
+}  // namespace
+
+// Wave -> (strip, band).  Blocks are spread round-robin over the 8 XCDs; the
+// remap gives each XCD a contiguous run of bands so that the 6-row aprons
+// shared by vertically adjacent bands are read through one L2.
+template <int FMT, int PAT, int MODE, int TH>
+__global__ __launch_bounds__(256) void k_fast(FastParams f) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    const int nblk = gridDim.x;
+    const int per_xcd = (nblk + 7) / 8;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int lblk = xcd * per_xcd + slot;  // logical block: consecutive within an XCD
+    const int wave = lblk * wpb + (threadIdx.x >> 6);
+    if (lblk >= nblk || wave >= f.nwaves) return;
+    const int by = wave % f.nby, sx = wave / f.nby;  // band-fastest: a block walks down a strip
+    const int x = sx * 248 - 4 + 4 * lane;
+    const int y0 = by * TH;
+    const bool lane_out = lane >= 1 && lane <= 62 && x < f.w;
+    const int xs0 = sx * 248, xs1 = xs0 + 248;
+    // careful: the band's outputs touch the 3-pixel frame rim, or its raw
+    // window reaches the stream's final row
+    const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
+    if (careful) run_band<FMT, PAT, MODE, true, TH>(f, x, y0, lane_out);
+    else run_band<FMT, PAT, MODE, false, TH>(f, x, y0, lane_out);
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int FMT, int PAT, int MODE>
+static hipError_t launch_fmp(const FastParams& f, hipStream_t st) {
+    const int waves_per_block = 4;
+    const int blocks = (f.nwaves + waves_per_block - 1) / waves_per_block;
+    if (f.th == 8) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8>), dim3(blocks), dim3(64 * waves_per_block), 0, st, f);
+    else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16>), dim3(blocks), dim3(64 * waves_per_block), 0, st, f);
+    return hipGetLastError();
+}
+
+template <int FMT, int PAT>
+static hipError_t launch_fp(const FastParams& f, int mode, hipStream_t st) {
+    switch (mode) {
+    case MODE_HQLI: return launch_fmp<FMT, PAT, MODE_HQLI>(f, st);
+    case MODE_MG_EXACT: return launch_fmp<FMT, PAT, MODE_MG_EXACT>(f, st);
+    default: return launch_fmp<FMT, PAT, MODE_MG_INT8>(f, st);
+    }
+}
+
+template <int FMT>
+static hipError_t launch_f(const FastParams& f, int mode, hipStream_t st) {
+    switch (f.pattern) {
+    case 0: return launch_fp<FMT, 0>(f, mode, st);
+    case 1: return launch_fp<FMT, 1>(f, mode, st);
+    case 2: return launch_fp<FMT, 2>(f, mode, st);
+    default: return launch_fp<FMT, 3>(f, mode, st);
+    }
+}
+
+}  // namespace imop_hip

@@ -1,0 +1,58 @@
+// fast_kernels.hip -- host-side selection and parameters of the
+// register-resident HQLI / MG kernel (fast_impl.hpp, instantiated per raw
+// layout in fast_u8 / fast_u16 / fast_p12 / fast_p10.hip).
+#include "device_common.hpp"
+#include "plain.hpp"
+
+namespace imop_hip {
+
+hipError_t launch_fast_u8(const FastParams& f, int mode, hipStream_t st);
+hipError_t launch_fast_u16(const FastParams& f, int mode, hipStream_t st);
+hipError_t launch_fast_p12(const FastParams& f, int mode, hipStream_t st);
+hipError_t launch_fast_p10(const FastParams& f, int mode, hipStream_t st);
+
+bool fast_path_ok(const FrameParams& f) {
+    if (f.w % 4 != 0 || f.w < 8 || f.h < 6) return false;
+    if (((uintptr_t)f.in | (uintptr_t)f.out) & 7) return false;
+    if ((uint64_t)f.w * f.h * 6u >= (1ull << 31) || f.in_bytes >= (1ull << 31)) return false;  // 32-bit offsets
+    if (f.fmt != FMT_U8 && f.fmt != FMT_U16 && f.fmt != FMT_P12 && f.fmt != FMT_P10) return false;
+    if (f.bl_on) {
+        for (int k = 0; k < 4; ++k) {
+            const float o = f.bl_q[k];
+            if (!(o >= 0.0f && o <= 65535.0f) || o != (float)(int)o) return false;
+        }
+    }
+    return true;
+}
+
+FastParams make_fast(const FrameParams& f, int th) {
+    FastParams q{};
+    q.in = f.in;
+    q.out = f.out;
+    q.in_bytes = f.in_bytes;
+    q.w = f.w;
+    q.h = f.h;
+    q.bd = f.bd;
+    q.maxv = f.maxv;
+    q.clamp_hi = f.clamp_hi;
+    q.pattern = f.pattern;
+    q.bl_on = f.bl_on;
+    for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
+    q.th = th == 8 ? 8 : 16;
+    q.nsx = (f.w + 247) / 248;
+    q.nby = (f.h + q.th - 1) / q.th;
+    q.nwaves = q.nsx * q.nby;
+    return q;
+}
+
+hipError_t launch_fast(const FrameParams& f, int mode, int th, hipStream_t st) {
+    FastParams q = make_fast(f, th);
+    switch (f.fmt) {
+    case FMT_U8: return launch_fast_u8(q, mode, st);
+    case FMT_U16: return launch_fast_u16(q, mode, st);
+    case FMT_P12: return launch_fast_p12(q, mode, st);
+    default: return launch_fast_p10(q, mode, st);
+    }
+}
+
+}  // namespace imop_hip

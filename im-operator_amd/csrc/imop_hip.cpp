@@ -143,6 +143,8 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
     int e = check_args(d_in, d_out, w, h, pattern, algo, bd, packed, bl, bl_mode);
     if (e != kOk) return e;
     if (in_bytes < bayer_bytes(w, h, bd, packed != 0) || out_bytes < rgb_bytes(w, h, bd)) return kInternal;
+    // u16 samples and u16 RGB are accessed as naturally aligned 16-bit words
+    if (bd > 8 && ((!packed && ((uintptr_t)d_in & 1)) || ((uintptr_t)d_out & 1))) return kInternal;
     int mode = -1;
     FrameParams f = make_params(d_in, in_bytes, d_out, w, h, pattern, bd, packed, bl, bl_mode, algo, &mode);
     hipError_t he;
@@ -151,6 +153,8 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
         he = get_scratch(prism_workspace_bytes(w, h), &ws);
         if (he != hipSuccess) return kInternal;
         he = launch_prism(f, ws, st);
+    } else if (fast_path_ok(f)) {
+        he = launch_fast(f, mode, 16, st);
     } else {
         he = launch_plain(f, mode, st);
     }

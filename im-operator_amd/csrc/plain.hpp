@@ -9,6 +9,11 @@ namespace imop_hip {
 // HQLI / MG tiled kernel for one frame (plain_kernels.hip).
 hipError_t launch_plain(const FrameParams& f, int mode, hipStream_t st);
 
+// Register-resident HQLI / MG kernel (fast_kernels.hip): usable when
+// fast_path_ok(); th = rows per wave (even).
+bool fast_path_ok(const FrameParams& f);
+hipError_t launch_fast(const FrameParams& f, int mode, int th, hipStream_t st);
+
 // In-place black level on an unpacked frame; q = offsets by (y & 1) * 2 + (x & 1).
 hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q[4], hipStream_t st);
 

@@ -87,9 +87,8 @@ __device__ void mg_pre_rgb(const FrameParams& f, int pat, int mode, int x, int y
                 if (s == 0) { r32 = 32 * p; b32 = other; } else { b32 = 32 * p; r32 = other; }
             } else {
                 int rs = 0, bs = 0;
-                const int dx[4] = {0, 0, -1, 1}, dy[4] = {-1, 1, 0, 0};
                 for (int k = 0; k < 4; ++k) {
-                    int xx = x + dx[k], yy = y + dy[k];
+                    int xx = x + (k == 2 ? -1 : k == 3 ? 1 : 0), yy = y + (k == 0 ? -1 : k == 1 ? 1 : 0);
                     int t = site_of(pat, yy, xx);
                     int d = 8 * sample_at(f, xx, yy) - mg_g8(f, pat, xx, yy);
                     if (t == 0) rs += d; else if (t == 2) bs += d;
@@ -116,9 +115,8 @@ __device__ void mg_pre_rgb(const FrameParams& f, int pat, int mode, int x, int y
                 if (s == 0) { rr = p; bb = other; } else { bb = p; rr = other; }
             } else {
                 int rs = 0, bs = 0;
-                const int dx[4] = {0, 0, -1, 1}, dy[4] = {-1, 1, 0, 0};
                 for (int k = 0; k < 4; ++k) {
-                    int xx = x + dx[k], yy = y + dy[k];
+                    int xx = x + (k == 2 ? -1 : k == 3 ? 1 : 0), yy = y + (k == 0 ? -1 : k == 1 ? 1 : 0);
                     int t = site_of(pat, yy, xx);
                     int d = sample_at(f, xx, yy) - mg_gi(f, pat, xx, yy);
                     if (t == 0) rs += d; else if (t == 2) bs += d;
@@ -199,6 +197,57 @@ __device__ __forceinline__ void store_pair(const FrameParams& f, int x, int y, i
     }
 }
 
+// One output pixel of a quad from the LDS tiles: sample plane `sp` (apron A)
+// and, for MG, colour-difference plane `sd` (apron 1).  (a, bb) is the
+// pixel's offset inside the quad, a compile-time constant after unrolling.
+template <int PAT, int MODE, int A, int LW, int DW>
+__device__ __forceinline__ void quad_px(const int* __restrict__ sp, const int* __restrict__ sd, int lq, int qx, int a,
+                                        int bb, int clamp_hi, int& r, int& g, int& b) {
+    const int s = site_of(PAT, a, bb);
+    const int* c = sp + (2 * lq + a + A) * LW + (2 * qx + bb + A);
+    if (MODE == MODE_HQLI) {
+        if (s != 1) {
+            int cross = c[-1] + c[1] + c[-LW] + c[LW];
+            int diag = c[-LW - 1] + c[-LW + 1] + c[LW - 1] + c[LW + 1];
+            int gg = (cross + 2) >> 2, oth = (diag + 2) >> 2;
+            if (s == 0) { r = c[0]; g = gg; b = oth; } else { b = c[0]; g = gg; r = oth; }
+        } else {
+            int hz = (c[-1] + c[1] + 1) >> 1, vt = (c[-LW] + c[LW] + 1) >> 1;
+            g = c[0];
+            if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
+        }
+        r = imin(r, clamp_hi); g = imin(g, clamp_hi); b = imin(b, clamp_hi);
+    } else {
+        const int* d = sd + (2 * lq + a + 1) * DW + (2 * qx + bb + 1);
+        const int p = c[0];
+        if (s != 1) {
+            const int dsum = d[-DW - 1] + d[-DW + 1] + d[DW - 1] + d[DW + 1];
+            int gg, oth;
+            if (MODE == MODE_MG_EXACT) {
+                gg = (8 * p - d[0] + 4) >> 3;
+                oth = (32 * p - 4 * d[0] + dsum + 16) >> 5;
+            } else {
+                gg = p - d[0];
+                oth = gg + tdiv4(dsum + 2);
+            }
+            if (s == 0) { r = p; g = gg; b = oth; } else { b = p; g = gg; r = oth; }
+        } else {
+            int hs = d[-1] + d[1], vs = d[-DW] + d[DW];
+            int hz, vt;
+            if (MODE == MODE_MG_EXACT) {
+                hz = (16 * p + hs + 8) >> 4;
+                vt = (16 * p + vs + 8) >> 4;
+            } else {
+                hz = p + tdiv2(hs + 1);
+                vt = p + tdiv2(vs + 1);
+            }
+            g = p;
+            if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
+        }
+        r = iclamp(r, 0, clamp_hi); g = iclamp(g, 0, clamp_hi); b = iclamp(b, 0, clamp_hi);
+    }
+}
+
 template <int PAT, int MODE, int QPT>
 __global__ __launch_bounds__(256) void k_plain_tile(FrameParams f, int tiles_x) {
     constexpr int TW = 128, TH = 8 * QPT;
@@ -237,80 +286,52 @@ __global__ __launch_bounds__(256) void k_plain_tile(FrameParams f, int tiles_x) 
 
     const int qx = threadIdx.x & 63;
     const int qrow = threadIdx.x >> 6;
-    const bool even_w = (w & 1) == 0;
+    const bool pair_ok = (w & 1) == 0 && (((uintptr_t)f.out & 3) == 0);
     for (int j = 0; j < QPT; ++j) {
         const int lq = j * 4 + qrow;  // quad row within the tile
         const int px = x0 + 2 * qx, py = y0 + 2 * lq;
         if (px >= w || py >= h) continue;
-        int out[2][2][3];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb) {
-                const int s = site_of(PAT, a, bb);
-                const int* c = sp + (2 * lq + a + A) * LW + (2 * qx + bb + A);
-                int r, g, b;
-                if (MODE == MODE_HQLI) {
-                    if (s != 1) {
-                        int cross = c[-1] + c[1] + c[-LW] + c[LW];
-                        int diag = c[-LW - 1] + c[-LW + 1] + c[LW - 1] + c[LW + 1];
-                        int gg = (cross + 2) >> 2, oth = (diag + 2) >> 2;
-                        if (s == 0) { r = c[0]; g = gg; b = oth; } else { b = c[0]; g = gg; r = oth; }
-                    } else {
-                        int hz = (c[-1] + c[1] + 1) >> 1, vt = (c[-LW] + c[LW] + 1) >> 1;
-                        g = c[0];
-                        if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
-                    }
-                    r = imin(r, f.clamp_hi); g = imin(g, f.clamp_hi); b = imin(b, f.clamp_hi);
-                } else {
-                    const int* d = sd + (2 * lq + a + 1) * DW + (2 * qx + bb + 1);
-                    const int p = c[0];
-                    if (s != 1) {
-                        const int dsum = d[-DW - 1] + d[-DW + 1] + d[DW - 1] + d[DW + 1];
-                        int own = p, gg, oth;
-                        if (MODE == MODE_MG_EXACT) {
-                            gg = (8 * p - d[0] + 4) >> 3;
-                            oth = (32 * p - 4 * d[0] + dsum + 16) >> 5;
-                        } else {
-                            gg = p - d[0];
-                            oth = gg + tdiv4(dsum + 2);
-                        }
-                        if (s == 0) { r = own; g = gg; b = oth; } else { b = own; g = gg; r = oth; }
-                    } else {
-                        int hs = d[-1] + d[1], vs = d[-DW] + d[DW];
-                        int hz, vt;
-                        if (MODE == MODE_MG_EXACT) {
-                            hz = (16 * p + hs + 8) >> 4;
-                            vt = (16 * p + vs + 8) >> 4;
-                        } else {
-                            hz = p + tdiv2(hs + 1);
-                            vt = p + tdiv2(vs + 1);
-                        }
-                        g = p;
-                        if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
-                    }
-                    r = iclamp(r, 0, f.clamp_hi); g = iclamp(g, 0, f.clamp_hi); b = iclamp(b, 0, f.clamp_hi);
-                }
-                out[a][bb][0] = r; out[a][bb][1] = g; out[a][bb][2] = b;
-            }
-        }
-        for (int a = 0; a < 2; ++a) {
+            int r0, g0, b0, r1, g1, b1;
+            quad_px<PAT, MODE, A, LW, DW>(sp, sd, lq, qx, a, 0, f.clamp_hi, r0, g0, b0);
+            quad_px<PAT, MODE, A, LW, DW>(sp, sd, lq, qx, a, 1, f.clamp_hi, r1, g1, b1);
             const int y = py + a;
-            if (y >= h) break;
-            const bool in0 = g_interior(px, y, w, h), in1 = g_interior(px + 1, y, w, h);
-            if (in0 && in1 && even_w) {
-                store_pair(f, px, y, out[a][0][0], out[a][0][1], out[a][0][2], out[a][1][0], out[a][1][1],
-                           out[a][1][2]);
-            } else {
-                if (in0) put_px(f, px, y, out[a][0][0], out[a][0][1], out[a][0][2]);
-                else border_pixel(f, PAT, MODE, px, y);
-                if (px + 1 < w) {
-                    if (in1) put_px(f, px + 1, y, out[a][1][0], out[a][1][1], out[a][1][2]);
-                    else border_pixel(f, PAT, MODE, px + 1, y);
+            if (y < h) {
+                // The outer two rows / columns are written by k_plain_border.
+                const bool in0 = g_interior(px, y, w, h), in1 = g_interior(px + 1, y, w, h);
+                if (in0 && in1 && pair_ok) {
+                    store_pair(f, px, y, r0, g0, b0, r1, g1, b1);
+                } else {
+                    if (in0) put_px(f, px, y, r0, g0, b0);
+                    if (in1) put_px(f, px + 1, y, r1, g1, b1);
                 }
             }
         }
     }
+}
+
+// The outer ring (2 rows / columns) of a frame, one thread per pixel, read
+// straight from HBM.  Ring pixel k: the top two rows, the bottom two rows,
+// then the left and right two columns of the rows in between.
+template <int PAT, int MODE>
+__global__ __launch_bounds__(256) void k_plain_border(FrameParams f) {
+    const int w = f.w, h = f.h;
+    const long long ring = 4LL * w + 4LL * (h - 4);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ring) return;
+    int x, y;
+    if (k < 4LL * w) {
+        const int r = (int)(k / w);
+        x = (int)(k - (long long)r * w);
+        y = r < 2 ? r : h - 4 + r;
+    } else {
+        const long long m = k - 4LL * w;
+        const int r = (int)(m >> 2), c = (int)(m & 3);
+        y = 2 + r;
+        x = c < 2 ? c : w - 4 + c;
+    }
+    border_pixel(f, PAT, MODE, x, y);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -321,6 +342,8 @@ static hipError_t launch_pm(const FrameParams& f, hipStream_t st) {
     const int tiles_x = (f.w + 127) / 128;
     const int tiles_y = (f.h + 8 * QPT - 1) / (8 * QPT);
     hipLaunchKernelGGL((k_plain_tile<PAT, MODE, QPT>), dim3(tiles_x * tiles_y), dim3(256), 0, st, f, tiles_x);
+    const long long ring = 4LL * f.w + 4LL * (f.h - 4);
+    hipLaunchKernelGGL((k_plain_border<PAT, MODE>), dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, st, f);
     return hipGetLastError();
 }
 
