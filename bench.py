@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--workload", default="mg12", choices=sorted(WORKLOADS))
     ap.add_argument("--sets", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed loop, run 4 plain device copies of this many MiB "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -199,6 +202,14 @@ def main():
             "mean_launch_us": round(mean_launch_ms * 1e3, 3),
         },
     }
+    if args.copy_mb > 0:
+        n = args.copy_mb << 20
+        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty(n, dtype=torch.uint8, device=dev)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: {4} calibration copies of {n} bytes done")
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(raw, args.workload)

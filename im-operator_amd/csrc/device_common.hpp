@@ -51,6 +51,8 @@ struct FastParams {
     int32_t th;           // rows per wave (even)
     int32_t nsx, nby;     // strips (248 columns each) and bands
     int32_t nwaves;
+    int32_t wpb;          // waves per workgroup
+    int32_t xcd_remap;    // give each XCD a contiguous run of waves
 };
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }

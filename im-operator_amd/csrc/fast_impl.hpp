@@ -67,6 +67,24 @@ __device__ __forceinline__ uint32_t lane_offset(uint32_t x) {
     return FMT == FMT_U8 ? x : FMT == FMT_U16 ? 2u * x : FMT == FMT_P12 ? x / 2u * 3u : x / 4u * 5u;
 }
 
+// Edge bands: exact-length loads that never read past the group run, so
+// the stream's final row needs no special case (with w % 4 == 0 every
+// packed row ends on a whole group, and 12-bit runs start on an even byte).
+template <int FMT>
+__device__ __forceinline__ Raw load_raw_exact(const FastParams& f, uint32_t off, bool ok) {
+    Raw r;
+    const uint8_t* p = f.in + (ok ? off : 0u);
+    if (FMT == FMT_P12) {
+        const uint32_t a = *(const uint16_t*)p, b = *(const uint16_t*)(p + 2), c = *(const uint16_t*)(p + 4);
+        r.lo = a | (b << 16);
+        r.hi = c;
+    } else {
+        r.lo = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+        r.hi = p[4];
+    }
+    return r;
+}
+
 template <int FMT>
 __device__ __forceinline__ Raw load_raw(const FastParams& f, uint32_t off, bool ok) {
     Raw r;
@@ -123,21 +141,6 @@ __device__ __forceinline__ void decode4(const FastParams& f, Raw r, bool ok, int
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) p[k] = (!MASK || ok) ? (int)v[k] : 0;
-}
-
-// The stream's final row of a packed layout: per-sample decode that keeps
-// get_packed_raw's bound (a group touching the last byte reads 0).
-template <int FMT>
-__device__ __forceinline__ void decode_last_row(const FastParams& f, uint32_t y, uint32_t x, bool ok, int o0, int o1,
-                                                int p[4]) {
-    const uint64_t idx = (uint64_t)y * f.w + x;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int v = 0;
-        if (ok) v = (FMT == FMT_P12) ? decode_p12(f.in, idx + k, f.in_bytes) : decode_p10(f.in, idx + k, f.in_bytes);
-        if (f.bl_on) v = imax(v - ((k & 1) ? o1 : o0), 0);
-        p[k] = ok ? v : 0;
-    }
 }
 
 // Colour role of CFA position parity (py, px) for the pattern.
@@ -264,14 +267,17 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
     constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10;
 
     // issue every load of the band (interior bands: every row and lane is
-    // inside the frame, so no masking)
+    // inside the frame, so no masking; their 8-byte windows end before the
+    // stream does because the final row is never in their window)
     const uint32_t rs = row_stride<FMT>((uint32_t)w), lo = lane_offset<FMT>(xu);
     Raw raw[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const int y = y0 - 3 + i;
-        const bool row_ok = !CAREFUL || (y >= 0 && y < h && !(packed && y == h - 1));
-        raw[i] = load_raw<FMT>(f, (row_ok ? (uint32_t)y : 0u) * rs + lo, !CAREFUL || (row_ok && lane_ok));
+        const bool row_ok = !CAREFUL || (y >= 0 && y < h);
+        const uint32_t off = (row_ok ? (uint32_t)y : 0u) * rs + lo;
+        if (CAREFUL && packed) raw[i] = load_raw_exact<FMT>(f, off, row_ok && lane_ok);
+        else raw[i] = load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
     }
 
     int P[NR][4];
@@ -279,9 +285,7 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
         const int y = y0 - 3 + i;
         const int o0 = f.bl_i[(y & 1) * 2], o1 = f.bl_i[(y & 1) * 2 + 1];
         const bool row_ok = y >= 0 && y < h;
-        // only careful bands reach the stream's final row
-        if (CAREFUL && packed && y == h - 1) decode_last_row<FMT>(f, (uint32_t)y, xu, lane_ok, o0, o1, P[i]);
-        else decode4<FMT, CAREFUL>(f, raw[i], row_ok && lane_ok, o0, o1, P[i]);
+        decode4<FMT, CAREFUL>(f, raw[i], row_ok && lane_ok, o0, o1, P[i]);
     };
 
     int D[TH + 2][3];  // MG colour differences, rows y0-1 .. y0+TH
@@ -384,10 +388,15 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     const int nblk = gridDim.x;
     const int per_xcd = (nblk + 7) / 8;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int lblk = xcd * per_xcd + slot;  // logical block: consecutive within an XCD
+    // logical block: consecutive within an XCD (or the launch order)
+    const int lblk = (f.xcd_remap & 1) ? xcd * per_xcd + slot : (int)blockIdx.x;
     const int wave = lblk * wpb + (threadIdx.x >> 6);
     if (lblk >= nblk || wave >= f.nwaves) return;
-    const int by = wave % f.nby, sx = wave / f.nby;  // band-fastest: a block walks down a strip
+    // wave order: band-fastest (consecutive waves walk down a strip) or
+    // strip-fastest (consecutive waves cover a row band left to right)
+    const bool strip_fastest = (f.xcd_remap & 2) != 0;
+    const int by = strip_fastest ? wave / f.nsx : wave % f.nby;
+    const int sx = strip_fastest ? wave % f.nsx : wave / f.nby;
     const int x = sx * 248 - 4 + 4 * lane;
     const int y0 = by * TH;
     const bool lane_out = lane >= 1 && lane <= 62 && x < f.w;
@@ -403,10 +412,13 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
 
 template <int FMT, int PAT, int MODE>
 static hipError_t launch_fmp(const FastParams& f, hipStream_t st) {
-    const int waves_per_block = 4;
-    const int blocks = (f.nwaves + waves_per_block - 1) / waves_per_block;
-    if (f.th == 8) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8>), dim3(blocks), dim3(64 * waves_per_block), 0, st, f);
-    else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16>), dim3(blocks), dim3(64 * waves_per_block), 0, st, f);
+    const int blocks = (f.nwaves + f.wpb - 1) / f.wpb;
+    switch (f.th) {
+    case 2: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    case 4: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    case 8: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    default: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    }
     return hipGetLastError();
 }
 

@@ -1,10 +1,27 @@
 // fast_kernels.hip -- host-side selection and parameters of the
 // register-resident HQLI / MG kernel (fast_impl.hpp, instantiated per raw
 // layout in fast_u8 / fast_u16 / fast_p12 / fast_p10.hip).
+#include <atomic>
+
 #include "device_common.hpp"
 #include "plain.hpp"
 
 namespace imop_hip {
+
+// Launch shape (set by the autotuner through imop_hip_set_launch_shape).
+static std::atomic<int> g_rows{16}, g_wpb{4}, g_xcd{1};
+
+void set_launch_shape(int rows, int wpb, int xcd_remap) {
+    if (rows == 2 || rows == 4 || rows == 8 || rows == 16) g_rows = rows;
+    if (wpb == 1 || wpb == 2 || wpb == 4) g_wpb = wpb;
+    if (xcd_remap >= 0 && xcd_remap <= 3) g_xcd = xcd_remap;
+}
+
+void get_launch_shape(int* rows, int* wpb, int* xcd_remap) {
+    if (rows) *rows = g_rows;
+    if (wpb) *wpb = g_wpb;
+    if (xcd_remap) *xcd_remap = g_xcd;
+}
 
 hipError_t launch_fast_u8(const FastParams& f, int mode, hipStream_t st);
 hipError_t launch_fast_u16(const FastParams& f, int mode, hipStream_t st);
@@ -38,15 +55,17 @@ FastParams make_fast(const FrameParams& f, int th) {
     q.pattern = f.pattern;
     q.bl_on = f.bl_on;
     for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
-    q.th = th == 8 ? 8 : 16;
+    q.th = (th == 2 || th == 4 || th == 8) ? th : 16;
+    q.wpb = g_wpb;
+    q.xcd_remap = g_xcd;
     q.nsx = (f.w + 247) / 248;
     q.nby = (f.h + q.th - 1) / q.th;
     q.nwaves = q.nsx * q.nby;
     return q;
 }
 
-hipError_t launch_fast(const FrameParams& f, int mode, int th, hipStream_t st) {
-    FastParams q = make_fast(f, th);
+hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st) {
+    FastParams q = make_fast(f, g_rows);
     switch (f.fmt) {
     case FMT_U8: return launch_fast_u8(q, mode, st);
     case FMT_U16: return launch_fast_u16(q, mode, st);

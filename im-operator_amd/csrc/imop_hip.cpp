@@ -79,7 +79,11 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     FrameParams f{};
     f.in = (const uint8_t*)in;
     f.out = (uint8_t*)out;
-    f.in_bytes = in_bytes;
+    // Packed groups are bounded by the stream length pixel::
+    // compute_packed_byte_size gives (pixel_utils.hpp:65-66), not by the
+    // caller's buffer, which may be larger.
+    (void)in_bytes;
+    f.in_bytes = bayer_bytes(w, h, bd, packed != 0);
     f.w = w;
     f.h = h;
     f.bd = bd;
@@ -154,7 +158,7 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
         if (he != hipSuccess) return kInternal;
         he = launch_prism(f, ws, st);
     } else if (fast_path_ok(f)) {
-        he = launch_fast(f, mode, 16, st);
+        he = launch_fast(f, mode, st);
     } else {
         he = launch_plain(f, mode, st);
     }
@@ -261,6 +265,16 @@ int imop_hip_demosaic_host(const uint8_t* bayer, uint8_t* rgb, int width, int he
                      is_packed, black_level, black_level_mode, nullptr);
     if (e != kOk) return e;
     if (hipMemcpy(rgb, t_staging.d_out, ob, hipMemcpyDeviceToHost) != hipSuccess) return kInternal;
+    return kOk;
+}
+
+int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap) {
+    set_launch_shape(rows_per_wave, waves_per_block, xcd_remap);
+    return kOk;
+}
+
+int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap) {
+    get_launch_shape(rows_per_wave, waves_per_block, xcd_remap);
     return kOk;
 }
 

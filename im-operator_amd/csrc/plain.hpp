@@ -10,9 +10,11 @@ namespace imop_hip {
 hipError_t launch_plain(const FrameParams& f, int mode, hipStream_t st);
 
 // Register-resident HQLI / MG kernel (fast_kernels.hip): usable when
-// fast_path_ok(); th = rows per wave (even).
+// fast_path_ok(); launch shape from set_launch_shape().
 bool fast_path_ok(const FrameParams& f);
-hipError_t launch_fast(const FrameParams& f, int mode, int th, hipStream_t st);
+hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st);
+void set_launch_shape(int rows, int wpb, int xcd_remap);
+void get_launch_shape(int* rows, int* wpb, int* xcd_remap);
 
 // In-place black level on an unpacked frame; q = offsets by (y & 1) * 2 + (x & 1).
 hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q[4], hipStream_t st);

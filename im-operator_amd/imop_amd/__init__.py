@@ -90,6 +90,10 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_black_level.restype = i
         L.imop_hip_black_level_host.argtypes = [u8p, i, i, i, i, i, fp]
         L.imop_hip_black_level_host.restype = i
+        L.imop_hip_set_launch_shape.argtypes = [i, i, i]
+        L.imop_hip_set_launch_shape.restype = i
+        L.imop_hip_get_launch_shape.argtypes = [ctypes.POINTER(i)] * 3
+        L.imop_hip_get_launch_shape.restype = i
         _lib = L
     return _lib
 
@@ -178,3 +182,14 @@ def black_level(d_data, width: int, height: int, pattern: int, algorithm: int, b
     return lib().imop_hip_black_level(ctypes.c_void_p(d_data.data_ptr()), width, height, int(pattern),
                                       int(algorithm), bit_depth, o.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                       ctypes.c_void_p(stream))
+
+
+def set_launch_shape(rows_per_wave: int = 0, waves_per_block: int = 0, xcd_remap: int = -1) -> None:
+    """Launch shape of the register-resident kernel (see imop_hip.h)."""
+    lib().imop_hip_set_launch_shape(rows_per_wave, waves_per_block, xcd_remap)
+
+
+def get_launch_shape() -> tuple[int, int, int]:
+    r, w, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().imop_hip_get_launch_shape(ctypes.byref(r), ctypes.byref(w), ctypes.byref(x))
+    return r.value, w.value, x.value

@@ -1,0 +1,118 @@
+"""Autotuner for the launch shape of the register-resident HQLI / MG kernel.
+
+The shape is (rows each wave walks down, waves per workgroup, XCD remap):
+rows sets the vertical apron overhead (6 extra raw rows per band) against
+the number of waves in flight; waves per workgroup and the XCD remap set how
+bands are spread over the 8 XCDs and their L2s.  Each candidate is timed on
+device-resident synthetic frames of the requested class with HIP events,
+rotating enough buffer sets to defeat the 256 MB Infinity Cache, and the
+fastest is applied with imop_hip_set_launch_shape.
+
+    python -m imop_amd.autotune --width 4024 --height 3032 --bits 12 --packed --algo 2
+
+Results can be cached in a JSON file keyed by frame class.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+
+import numpy as np
+
+import imop_amd
+from imop_amd import scene
+
+CANDIDATES = list(itertools.product((2, 4, 8, 16), (1, 4), (0, 1, 2, 3)))
+
+
+def frame_class(w: int, h: int, bd: int, packed: bool, algo: int) -> str:
+    return f"{w}x{h}-bd{bd}{'p' if packed else 'u'}-a{algo}"
+
+
+def time_config(d_in, d_out, w, h, bd, packed, algo, bl, iters=30):
+    import torch
+    stream = torch.cuda.current_stream()
+    n = len(d_in)
+    for k in range(3):
+        imop_amd.demosaic(d_in[k % n], d_out[k % n], w, h, 0, algo, bd, packed, bl, stream=stream)
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for k in range(iters):
+        e = imop_amd.demosaic(d_in[k % n], d_out[k % n], w, h, 0, algo, bd, packed, bl, stream=stream)
+        if e != 0:
+            raise RuntimeError(f"demosaic returned {e}")
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / iters * 1e3  # us per frame
+
+
+def tune(w: int, h: int, bd: int, packed: bool, algo: int, black_level=None, seed: int = 0,
+         candidates=CANDIDATES, verbose: bool = True):
+    """Time every candidate shape on this frame class; apply and return the best
+    as (rows, waves_per_block, xcd_remap, us_per_frame, table)."""
+    import torch
+    vals = scene.make_scene(h, w, bd, 0, seed=seed)
+    raw = scene.raw_bytes(vals, bd, packed)
+    out_b = imop_amd.rgb_byte_size(w, h, bd)
+    per_set = raw.size + out_b
+    nsets = max(2, min(8, (600 << 20) // per_set + 1))
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(nsets)]
+    d_out = [torch.empty(out_b, dtype=torch.uint8, device=dev) for _ in range(nsets)]
+    table = []
+    for rows, wpb, xcd in candidates:
+        imop_amd.set_launch_shape(rows, wpb, xcd)
+        us = time_config(d_in, d_out, w, h, bd, packed, algo, black_level)
+        table.append((rows, wpb, xcd, us))
+        if verbose:
+            print(f"autotune {frame_class(w, h, bd, packed, algo)} rows={rows} wpb={wpb} xcd={xcd}: {us:.2f} us",
+                  flush=True)
+    best = min(table, key=lambda t: t[3])
+    imop_amd.set_launch_shape(best[0], best[1], best[2])
+    return best[0], best[1], best[2], best[3], table
+
+
+def load_cache(path: str) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except OSError:
+        return {}
+
+
+def apply_cached(path: str, w: int, h: int, bd: int, packed: bool, algo: int) -> bool:
+    """Apply a cached shape for this frame class; False if none is cached."""
+    c = load_cache(path).get(frame_class(w, h, bd, packed, algo))
+    if not c:
+        return False
+    imop_amd.set_launch_shape(c["rows"], c["waves_per_block"], c["xcd_remap"])
+    return True
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--width", type=int, default=4024)
+    ap.add_argument("--height", type=int, default=3032)
+    ap.add_argument("--bits", type=int, default=12)
+    ap.add_argument("--packed", action="store_true")
+    ap.add_argument("--algo", type=int, default=2)
+    ap.add_argument("--black-level", type=float, default=64.0)
+    ap.add_argument("--cache", default="")
+    a = ap.parse_args()
+    rows, wpb, xcd, us, table = tune(a.width, a.height, a.bits, a.packed, a.algo, [a.black_level] * 4)
+    print(json.dumps({"class": frame_class(a.width, a.height, a.bits, a.packed, a.algo), "rows": rows,
+                      "waves_per_block": wpb, "xcd_remap": xcd, "us": round(us, 3)}))
+    if a.cache:
+        c = load_cache(a.cache)
+        c[frame_class(a.width, a.height, a.bits, a.packed, a.algo)] = {
+            "rows": rows, "waves_per_block": wpb, "xcd_remap": xcd, "us": round(us, 3)}
+        os.makedirs(os.path.dirname(os.path.abspath(a.cache)), exist_ok=True)
+        with open(a.cache, "w") as f:
+            json.dump(c, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

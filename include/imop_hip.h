@@ -86,6 +86,18 @@ int imop_hip_black_level(void* d_data, int width, int height, int pattern, int a
 int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern, int algorithm, int bit_depth,
                               const float* offsets);
 
+/* Launch shape of the register-resident HQLI / MG kernel, chosen by the
+ * autotuner (im-operator_amd/imop_amd/autotune.py): rows of a frame each
+ * wave walks down (2, 4, 8 or 16), waves per workgroup (1, 2 or 4), and the
+ * wave order: bit 0 remaps workgroups so each XCD works on a contiguous run
+ * of waves (else launch order), bit 1 orders waves strip-fastest (adjacent
+ * waves cover adjacent columns of the same rows; else they walk down a
+ * 248-column strip).  Out-of-range values leave a setting
+ * unchanged.  Process-wide; has no counterpart in the reference (its CUDA
+ * launch shape is fixed at 32x16, src/algorithms/cuda_kernels.cu:1496). */
+int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
+int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap);
+
 #ifdef __cplusplus
 }
 #endif

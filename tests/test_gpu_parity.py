@@ -174,3 +174,21 @@ def test_register_path_and_lds_path_against_oracle(case):
     rgb2 = (out.view(np.uint16) if bd > 8 else out).reshape(h, w, 3)
     d = np.argwhere(rgb2 != ro)
     assert d.size == 0, f"lds path: {len(d)} samples differ, first {d[:4].tolist()}"
+
+
+@pytest.mark.parametrize("bd", [12, 10, 14])
+@pytest.mark.parametrize("algo", [imop_amd.DemosaicAlgorithm.HQLI, imop_amd.DemosaicAlgorithm.MG,
+                                  imop_amd.DemosaicAlgorithm.PRISM])
+def test_packed_stream_bound_ignores_larger_buffer(bd, algo):
+    """A group that runs past the packed stream (odd pixel counts) decodes as
+    0 (pixel_utils.hpp:69-98, bounded by compute_packed_byte_size), even when
+    the caller's buffer is longer and holds non-zero bytes there."""
+    w, h = 13, 11
+    rng = np.random.default_rng(bd)
+    vals = rng.integers(0, 1 << bd, size=(h, w)).astype(np.uint16)
+    raw = scene.raw_bytes(vals, bd, True)
+    padded = np.concatenate([raw, np.full(16, 0xFF, dtype=np.uint8)])
+    e, rgb = gpu_run(padded, w, h, 1, algo, bd, True)
+    assert e == 0
+    _, ro = O.demosaic(raw, w, h, 1, int(algo), bd, True)
+    np.testing.assert_array_equal(rgb, ro)
