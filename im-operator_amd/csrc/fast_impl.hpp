@@ -386,10 +386,12 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nblk = gridDim.x;
-    const int per_xcd = (nblk + 7) / 8;
+    // logical block: consecutive within an XCD (or the launch order).  With
+    // nblk = 8q + r, XCD x runs q + (x < r) blocks; the remap is a bijection
+    // of [0, nblk) for every nblk.
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    // logical block: consecutive within an XCD (or the launch order)
-    const int lblk = (f.xcd_remap & 1) ? xcd * per_xcd + slot : (int)blockIdx.x;
+    const int q8 = nblk >> 3, r8 = nblk & 7;
+    const int lblk = (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x;
     const int wave = lblk * wpb + (threadIdx.x >> 6);
     if (lblk >= nblk || wave >= f.nwaves) return;
     // wave order: band-fastest (consecutive waves walk down a strip) or

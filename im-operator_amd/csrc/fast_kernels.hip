@@ -9,7 +9,7 @@
 namespace imop_hip {
 
 // Launch shape (set by the autotuner through imop_hip_set_launch_shape).
-static std::atomic<int> g_rows{16}, g_wpb{4}, g_xcd{1};
+static std::atomic<int> g_rows{4}, g_wpb{4}, g_xcd{3};
 
 void set_launch_shape(int rows, int wpb, int xcd_remap) {
     if (rows == 2 || rows == 4 || rows == 8 || rows == 16) g_rows = rows;

@@ -24,7 +24,7 @@ import numpy as np
 import imop_amd
 from imop_amd import scene
 
-CANDIDATES = list(itertools.product((2, 4, 8, 16), (1, 4), (0, 1, 2, 3)))
+CANDIDATES = list(itertools.product((2, 4, 8, 16), (1, 2, 4), (0, 1, 2, 3)))
 
 
 def frame_class(w: int, h: int, bd: int, packed: bool, algo: int) -> str:
@@ -63,9 +63,17 @@ def tune(w: int, h: int, bd: int, packed: bool, algo: int, black_level=None, see
     d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(nsets)]
     d_out = [torch.empty(out_b, dtype=torch.uint8, device=dev) for _ in range(nsets)]
     table = []
+    ref = None
     for rows, wpb, xcd in candidates:
         imop_amd.set_launch_shape(rows, wpb, xcd)
+        for o in d_out:
+            o.zero_()
         us = time_config(d_in, d_out, w, h, bd, packed, algo, black_level)
+        # every shape must produce the same frame; a mismatch is a bug, never a trade-off
+        if ref is None:
+            ref = d_out[0].clone()
+        elif not torch.equal(d_out[0], ref):
+            raise RuntimeError(f"launch shape rows={rows} wpb={wpb} xcd={xcd} changed the output")
         table.append((rows, wpb, xcd, us))
         if verbose:
             print(f"autotune {frame_class(w, h, bd, packed, algo)} rows={rows} wpb={wpb} xcd={xcd}: {us:.2f} us",
