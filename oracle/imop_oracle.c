@@ -27,8 +27,10 @@
  * orc_set_mg8_scalar(1).
  *
  * Floating-point paths (MG > 8-bit, PRISM) keep the reference's operation
- * order and are compiled with -ffp-contract=off, matching ISO C++17 GCC
- * (CMAKE_CXX_EXTENSIONS OFF, CMakeLists.txt:32).
+ * order.  This file is compiled with -ffp-contract=off and the multiply-adds
+ * that GCC fuses in the reference build (-O3 -mavx2 -mfma; GCC contracts
+ * C++ by default even under -std=c++17) are written as explicit fmaf()
+ * calls, see mad() below.
  *
  * Pinning: every function here is checked in tests/test_oracle.py against
  * the reference's own known-answer tests, closed forms, the golden digests in
