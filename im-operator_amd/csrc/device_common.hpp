@@ -37,6 +37,7 @@ struct FrameParams {
     int32_t clamp_hi;     // final clamp of the integer paths
     int32_t bl_on;        // black level applied before demosaic
     int32_t pattern;      // BayerPattern, 0..3
+    int32_t out16;        // RGB written as u16 (bd > 8, or on request for 8-bit frames)
     float bl_q[4];        // offset by (y & 1) * 2 + (x & 1), already max(0, .)
 };
 
@@ -46,6 +47,7 @@ struct FastParams {
     uint8_t* out;
     uint64_t in_bytes;
     int32_t w, h, bd, maxv, clamp_hi, pattern;
+    int32_t out16;        // RGB written as u16
     int32_t bl_on;
     int32_t bl_i[4];      // integral black level by (y & 1) * 2 + (x & 1)
     int32_t th;           // rows per wave (even)
@@ -121,10 +123,10 @@ __host__ __device__ constexpr int site_of(int pat, int y, int x) {
          : (((y & 1) != r_row_of(pat)) && ((x & 1) != r_col_of(pat))) ? 2 : 1;
 }
 
-// Write one RGB pixel (u8 for bd <= 8, else u16).
+// Write one RGB pixel (u8, or u16 when out16).
 __device__ __forceinline__ void put_px(const FrameParams& f, int x, int y, int r, int g, int b) {
     const uint64_t i = ((uint64_t)y * (uint64_t)f.w + (uint64_t)x) * 3u;
-    if (f.bd <= 8) {
+    if (!f.out16) {
         f.out[i] = (uint8_t)r; f.out[i + 1] = (uint8_t)g; f.out[i + 2] = (uint8_t)b;
     } else {
         uint16_t* o = (uint16_t*)f.out;

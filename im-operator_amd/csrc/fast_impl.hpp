@@ -24,8 +24,8 @@
 // columns 2 / w-3.
 //
 // Preconditions (checked on the host, else the generic LDS path runs):
-// width % 4 == 0, layout u8 / u16 / 12-bit packed / 10-bit packed, black
-// level absent or integral per site.
+// width % 4 == 0 or 2, layout u8 / u16 / 12-bit packed / 10-bit packed,
+// black level absent or integral per site.
 #pragma once
 
 #include "device_common.hpp"
@@ -103,11 +103,62 @@ __device__ __forceinline__ Raw load_raw(const FastParams& f, uint32_t off, bool 
     return r;
 }
 
-// Decode a raw chunk to 4 samples; black level (integral offsets o0 / o1 for
-// even / odd columns of the row) or MSB alignment applied.
-template <int FMT, bool MASK>
-__device__ __forceinline__ void decode4(const FastParams& f, Raw r, bool ok, int o0, int o1, int p[4]) {
-    uint32_t v[4];
+// Frames with w % 4 == 2: rows start at any byte, so the group run of a
+// lane starts at the general byte offset of pixel i = y * w + x (x % 4 == 0,
+// so i % 4 is 0 on even rows and 2 on odd rows).  A 10-bit run that starts at
+// sub-pixel 2 begins 2 bytes into its group: [P2hi P3hi tails | P0hi P1hi .. tails].
+template <int FMT>
+__device__ __forceinline__ uint32_t run_offset(uint32_t w, uint32_t x, uint32_t y) {
+    const uint32_t i = y * w + x;
+    return FMT == FMT_U8 ? i : FMT == FMT_U16 ? 2u * i : FMT == FMT_P12 ? (i >> 1) * 3u : (i >> 2) * 5u + (i & 2u);
+}
+
+// Bytes off .. off+7 (u8: off .. off+3) from whole aligned words: reads at
+// most 11 bytes past off, so callers use it only where the stream continues
+// that far (every row but the last).
+template <int FMT>
+__device__ __forceinline__ Raw load_raw_any(const FastParams& f, uint32_t off) {
+    Raw r;
+    const uint32_t s = off & 3u;
+    const uint32_t* p = (const uint32_t*)(f.in + (off & ~3u));
+    if (FMT == FMT_U16) {  // off % 4 == 0
+        r.lo = p[0];
+        r.hi = p[1];
+    } else if (FMT == FMT_U8) {
+        const uint32_t d0 = p[0], d1 = p[1];
+        r.lo = __builtin_amdgcn_alignbyte(d1, d0, s);
+        r.hi = 0;
+    } else {
+        const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
+        r.lo = __builtin_amdgcn_alignbyte(d1, d0, s);
+        r.hi = __builtin_amdgcn_alignbyte(d2, d1, s);
+    }
+    return r;
+}
+
+// Raw value of pixel (x, y), x < w, through the bounded per-pixel decoders
+// (a group reaching past the stream decodes as 0): the last row of frames
+// with w % 4 == 2, whose final group may be cut short.
+template <int FMT>
+__device__ __forceinline__ uint32_t raw_pixel(const FastParams& f, uint32_t x, uint32_t y) {
+    const uint64_t i = (uint64_t)y * (uint32_t)f.w + x;
+    if (FMT == FMT_U8) return f.in[i];
+    if (FMT == FMT_U16) return ((const uint16_t*)f.in)[i];
+    if (FMT == FMT_P12) return (uint32_t)decode_p12(f.in, i, f.in_bytes);
+    return (uint32_t)decode_p10(f.in, i, f.in_bytes);
+}
+
+// Decode a raw chunk to 4 raw values.  sub2: a 10-bit run starting at
+// sub-pixel 2 (odd rows of w % 4 == 2 frames).
+template <int FMT>
+__device__ __forceinline__ void raw_values(Raw r, bool sub2, uint32_t v[4]) {
+    if (FMT == FMT_P10 && sub2) {
+        v[0] = ((r.lo & 0xFF) << 2) | ((r.lo >> 20) & 3);
+        v[1] = (((r.lo >> 8) & 0xFF) << 2) | ((r.lo >> 22) & 3);
+        v[2] = ((r.lo >> 24) << 2) | ((r.hi >> 24) & 3);
+        v[3] = ((r.hi & 0xFF) << 2) | ((r.hi >> 26) & 3);
+        return;
+    }
     if (FMT == FMT_U8) {
         v[0] = r.lo & 0xFF; v[1] = (r.lo >> 8) & 0xFF; v[2] = (r.lo >> 16) & 0xFF; v[3] = r.lo >> 24;
     } else if (FMT == FMT_U16) {
@@ -125,6 +176,12 @@ __device__ __forceinline__ void decode4(const FastParams& f, Raw r, bool ok, int
         v[2] = (((r.lo >> 16) & 0xFF) << 2) | ((t >> 4) & 3);
         v[3] = ((r.lo >> 24) << 2) | (t >> 6);
     }
+}
+
+// Black level (integral offsets o0 / o1 for even / odd columns of the row)
+// or MSB alignment applied to 4 raw values.
+template <int FMT, bool MASK>
+__device__ __forceinline__ void finish4(const FastParams& f, uint32_t v[4], bool ok, int o0, int o1, int p[4]) {
     if (FMT != FMT_U16) {
         // decoded samples never exceed max; offsets are 0 without black level
 #pragma unroll
@@ -151,20 +208,45 @@ __device__ constexpr int role(int py, int px) { return site_of(PAT, py, px); }
 template <int PAT>
 __device__ constexpr int rb_col_parity(int py) { return py == r_row_of(PAT) ? r_col_of(PAT) : 1 - r_col_of(PAT); }
 
-// Store 4 RGB pixels of one row at (x, y) (x % 4 == 0); 32-bit offsets.
+// Store the RGB pixels of one lane run at (x, y) (x % 4 == 0); 32-bit
+// offsets.  ODD (w % 4 == 2): u16 runs are 4-byte aligned and u8 runs on odd
+// rows only 2-byte aligned, and the last lane of a row holds 2 pixels.
+template <bool ODD>
 __device__ __forceinline__ void store4(const FastParams& f, uint32_t x, uint32_t y, const int r[4], const int g[4],
                                        const int b[4]) {
     const uint32_t i = (y * (uint32_t)f.w + x) * 3u;
-    if (f.bd > 8) {
-        uint2* o = (uint2*)(f.out + 2u * i);
-        o[0] = make_uint2((uint32_t)r[0] | ((uint32_t)g[0] << 16), (uint32_t)b[0] | ((uint32_t)r[1] << 16));
-        o[1] = make_uint2((uint32_t)g[1] | ((uint32_t)b[1] << 16), (uint32_t)r[2] | ((uint32_t)g[2] << 16));
-        o[2] = make_uint2((uint32_t)b[2] | ((uint32_t)r[3] << 16), (uint32_t)g[3] | ((uint32_t)b[3] << 16));
+    const bool full = !ODD || x + 4u <= (uint32_t)f.w;
+    if (f.out16) {
+        const uint32_t w0 = (uint32_t)r[0] | ((uint32_t)g[0] << 16), w1 = (uint32_t)b[0] | ((uint32_t)r[1] << 16);
+        const uint32_t w2 = (uint32_t)g[1] | ((uint32_t)b[1] << 16), w3 = (uint32_t)r[2] | ((uint32_t)g[2] << 16);
+        const uint32_t w4 = (uint32_t)b[2] | ((uint32_t)r[3] << 16), w5 = (uint32_t)g[3] | ((uint32_t)b[3] << 16);
+        if (!ODD) {
+            uint2* o = (uint2*)(f.out + 2u * i);
+            o[0] = make_uint2(w0, w1);
+            o[1] = make_uint2(w2, w3);
+            o[2] = make_uint2(w4, w5);
+        } else {
+            uint32_t* o = (uint32_t*)(f.out + 2u * i);
+            o[0] = w0; o[1] = w1; o[2] = w2;
+            if (full) { o[3] = w3; o[4] = w4; o[5] = w5; }
+        }
     } else {
-        uint32_t* o = (uint32_t*)(f.out + i);
-        o[0] = (uint32_t)r[0] | ((uint32_t)g[0] << 8) | ((uint32_t)b[0] << 16) | ((uint32_t)r[1] << 24);
-        o[1] = (uint32_t)g[1] | ((uint32_t)b[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)g[2] << 24);
-        o[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
+        const uint32_t w0 = (uint32_t)r[0] | ((uint32_t)g[0] << 8) | ((uint32_t)b[0] << 16) | ((uint32_t)r[1] << 24);
+        const uint32_t w1 = (uint32_t)g[1] | ((uint32_t)b[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)g[2] << 24);
+        const uint32_t w2 = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
+        if (!ODD) {
+            uint32_t* o = (uint32_t*)(f.out + i);
+            o[0] = w0; o[1] = w1; o[2] = w2;
+        } else if ((y & 1u) == 0) {  // 4-byte aligned
+            uint32_t* o = (uint32_t*)(f.out + i);
+            o[0] = w0;
+            if (full) { o[1] = w1; o[2] = w2; }
+            else *(uint16_t*)(f.out + i + 4) = (uint16_t)w1;
+        } else {                     // 2-byte aligned
+            uint16_t* o = (uint16_t*)(f.out + i);
+            o[0] = (uint16_t)w0; o[1] = (uint16_t)(w0 >> 16); o[2] = (uint16_t)w1;
+            if (full) { o[3] = (uint16_t)(w1 >> 16); o[4] = (uint16_t)w2; o[5] = (uint16_t)(w2 >> 16); }
+        }
     }
 }
 
@@ -251,7 +333,8 @@ __device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const 
     }
 }
 
-template <int FMT, int PAT, int MODE, bool CAREFUL, int TH>
+// One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD>
 __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, bool lane_out) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
     const int w = f.w, h = f.h;
@@ -275,6 +358,13 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
     for (int i = 0; i < NR; ++i) {
         const int y = y0 - 3 + i;
         const bool row_ok = !CAREFUL || (y >= 0 && y < h);
+        if (ODD) {
+            // the last row is decoded pixel by pixel (its final group may be
+            // cut short); every other row has >= 11 stream bytes past the run
+            const bool ok = row_ok && lane_ok && !(CAREFUL && y == h - 1);
+            raw[i] = load_raw_any<FMT>(f, ok ? run_offset<FMT>((uint32_t)w, xu, (uint32_t)y) : 0u);
+            continue;
+        }
         const uint32_t off = (row_ok ? (uint32_t)y : 0u) * rs + lo;
         if (CAREFUL && packed) raw[i] = load_raw_exact<FMT>(f, off, row_ok && lane_ok);
         else raw[i] = load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
@@ -285,7 +375,14 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
         const int y = y0 - 3 + i;
         const int o0 = f.bl_i[(y & 1) * 2], o1 = f.bl_i[(y & 1) * 2 + 1];
         const bool row_ok = y >= 0 && y < h;
-        decode4<FMT, CAREFUL>(f, raw[i], row_ok && lane_ok, o0, o1, P[i]);
+        uint32_t v[4];
+        if (ODD && CAREFUL && y == h - 1) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
+        } else {
+            raw_values<FMT>(raw[i], ODD && ((i + 1) & 1), v);  // y0 even: row y is odd when i is even
+        }
+        finish4<FMT, CAREFUL>(f, v, row_ok && lane_ok, o0, o1, P[i]);
     };
 
     int D[TH + 2][3];  // MG colour differences, rows y0-1 .. y0+TH
@@ -348,12 +445,23 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
                 R[a][k] = r; G[a][k] = g; B[a][k] = b;
             }
         }
+        // w % 4 == 2: column w-3 of the right-edge column fill lives in the
+        // lane to the left (DPP while every lane is active)
+        int LR[2] = {0, 0}, LG[2] = {0, 0}, LB[2] = {0, 0};
+        if (ODD && CAREFUL) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                LR[a] = from_left(R[a][3]);
+                LG[a] = from_left(G[a][3]);
+                LB[a] = from_left(B[a][3]);
+            }
+        }
 
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int y = yA + a;
             if (!CAREFUL) {
-                if (lane_out) store4(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
+                if (lane_out) store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
                 continue;
             }
             if (y >= h || y < 2 || y >= h - 2 || !lane_out) continue;
@@ -366,12 +474,13 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
                 for (int k = 0; k < 4; ++k)
                     if (!col_ok[k]) { r[k] = 0; g[k] = 0; b[k] = 0; }
             }
-            if (y == 2) { store4(f, x, 0, r, g, b); store4(f, x, 1, r, g, b); }
-            if (y == h - 3) { store4(f, x, h - 2, r, g, b); store4(f, x, h - 1, r, g, b); }
+            if (y == 2) { store4<ODD>(f, x, 0, r, g, b); store4<ODD>(f, x, 1, r, g, b); }
+            if (y == h - 3) { store4<ODD>(f, x, h - 2, r, g, b); store4<ODD>(f, x, h - 1, r, g, b); }
             // column fill of the middle rows: columns 0-1 <- 2, w-2..w-1 <- w-3
             if (x == 0) { r[0] = r[1] = R[a][2]; g[0] = g[1] = G[a][2]; b[0] = b[1] = B[a][2]; }
-            if (x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
-            store4(f, x, y, r, g, b);
+            if (!ODD && x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
+            if (ODD && x == w - 2) { r[0] = r[1] = LR[a]; g[0] = g[1] = LG[a]; b[0] = b[1] = LB[a]; }
+            store4<ODD>(f, x, y, r, g, b);
         }
     }
 }
@@ -381,7 +490,7 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
 // Wave -> (strip, band).  Blocks are spread round-robin over the 8 XCDs; the
 // remap gives each XCD a contiguous run of bands so that the 6-row aprons
 // shared by vertically adjacent bands are read through one L2.
-template <int FMT, int PAT, int MODE, int TH>
+template <int FMT, int PAT, int MODE, int TH, bool ODD>
 __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
@@ -406,21 +515,27 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     // careful: the band's outputs touch the 3-pixel frame rim, or its raw
     // window reaches the stream's final row
     const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
-    if (careful) run_band<FMT, PAT, MODE, true, TH>(f, x, y0, lane_out);
-    else run_band<FMT, PAT, MODE, false, TH>(f, x, y0, lane_out);
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out);
+    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out);
 }
 
 // ---------------------------------------------------------------- launchers
 
-template <int FMT, int PAT, int MODE>
-static hipError_t launch_fmp(const FastParams& f, hipStream_t st) {
+template <int FMT, int PAT, int MODE, bool ODD>
+static void launch_fmpo(const FastParams& f, hipStream_t st) {
     const int blocks = (f.nwaves + f.wpb - 1) / f.wpb;
     switch (f.th) {
-    case 2: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    case 4: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    case 8: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    default: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    case 2: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    case 4: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    case 8: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    default: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
     }
+}
+
+template <int FMT, int PAT, int MODE>
+static hipError_t launch_fmp(const FastParams& f, hipStream_t st) {
+    if (f.w % 4 == 2) launch_fmpo<FMT, PAT, MODE, true>(f, st);
+    else launch_fmpo<FMT, PAT, MODE, false>(f, st);
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ hipError_t launch_fast_p12(const FastParams& f, int mode, hipStream_t st);
 hipError_t launch_fast_p10(const FastParams& f, int mode, hipStream_t st);
 
 bool fast_path_ok(const FrameParams& f) {
-    if (f.w % 4 != 0 || f.w < 8 || f.h < 6) return false;
+    if ((f.w % 4 != 0 && f.w % 4 != 2) || f.w < 8 || f.h < 6) return false;
     if (((uintptr_t)f.in | (uintptr_t)f.out) & 7) return false;
     if ((uint64_t)f.w * f.h * 6u >= (1ull << 31) || f.in_bytes >= (1ull << 31)) return false;  // 32-bit offsets
     if (f.fmt != FMT_U8 && f.fmt != FMT_U16 && f.fmt != FMT_P12 && f.fmt != FMT_P10) return false;
@@ -53,6 +53,7 @@ FastParams make_fast(const FrameParams& f, int th) {
     q.maxv = f.maxv;
     q.clamp_hi = f.clamp_hi;
     q.pattern = f.pattern;
+    q.out16 = f.out16;
     q.bl_on = f.bl_on;
     for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
     q.th = (th == 2 || th == 4 || th == 8) ? th : 16;

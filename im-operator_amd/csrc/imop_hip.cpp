@@ -88,6 +88,7 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     f.h = h;
     f.bd = bd;
     f.pattern = pattern;
+    f.out16 = bd > 8;
     f.maxv = (int)((1u << bd) - 1u);
     if (bd <= 8) f.fmt = FMT_U8;
     else if (!packed) f.fmt = FMT_U16;

@@ -184,7 +184,7 @@ __device__ __forceinline__ void fill_tile(const FrameParams& f, int* __restrict_
 __device__ __forceinline__ void store_pair(const FrameParams& f, int x, int y, int r0, int g0, int b0, int r1,
                                            int g1, int b1) {
     const uint64_t i = ((uint64_t)y * (uint64_t)f.w + (uint64_t)x) * 3u;
-    if (f.bd > 8) {
+    if (f.out16) {
         uint32_t* o = (uint32_t*)((uint16_t*)f.out + i);
         o[0] = (uint32_t)r0 | ((uint32_t)g0 << 16);
         o[1] = (uint32_t)b0 | ((uint32_t)r1 << 16);

@@ -134,8 +134,9 @@ def test_full_size_all_patterns_against_oracle(w, h):
 def _fast_eligible_cases():
     rng = np.random.default_rng(4242)
     out = []
-    widths = [8, 12, 16, 20, 36, 124, 244, 248, 252, 256, 260, 496, 500, 504, 748, 1000]
-    for t in range(120):
+    widths = [8, 12, 16, 20, 36, 124, 244, 248, 252, 256, 260, 496, 500, 504, 748, 1000,
+              10, 14, 18, 38, 126, 246, 250, 254, 258, 494, 498, 502, 506, 1006]
+    for t in range(180):
         w = int(widths[t % len(widths)])
         h = int(rng.integers(6, 90))
         algo = (imop_amd.DemosaicAlgorithm.HQLI, imop_amd.DemosaicAlgorithm.MG)[t % 2]
@@ -147,9 +148,9 @@ def _fast_eligible_cases():
 
 @pytest.mark.parametrize("case", _fast_eligible_cases(), ids=lambda c: f"{c[0]}x{c[1]}-a{int(c[2])}-bd{c[3]}{'p' if c[4] else ''}-pat{c[5]}-bl{c[6] is not None}")
 def test_register_path_and_lds_path_against_oracle(case):
-    """Widths divisible by 4 with aligned buffers take the register-resident
-    kernel; the same frame with the input 2 bytes off an 8-byte boundary
-    takes the LDS-tiled kernel.  Both must equal the oracle."""
+    """Even widths (w % 4 == 0 or 2) with aligned buffers take the
+    register-resident kernel; the same frame with the input 2 bytes off an
+    8-byte boundary takes the LDS-tiled kernel.  Both must equal the oracle."""
     w, h, algo, bd, packed, pat, bl, seed = case
     rng = np.random.default_rng(seed)
     vals = rng.integers(0, 1 << bd, size=(h, w)).astype(np.uint8 if bd <= 8 else np.uint16)
@@ -179,11 +180,13 @@ def test_register_path_and_lds_path_against_oracle(case):
 @pytest.mark.parametrize("bd", [12, 10, 14])
 @pytest.mark.parametrize("algo", [imop_amd.DemosaicAlgorithm.HQLI, imop_amd.DemosaicAlgorithm.MG,
                                   imop_amd.DemosaicAlgorithm.PRISM])
-def test_packed_stream_bound_ignores_larger_buffer(bd, algo):
-    """A group that runs past the packed stream (odd pixel counts) decodes as
-    0 (pixel_utils.hpp:69-98, bounded by compute_packed_byte_size), even when
-    the caller's buffer is longer and holds non-zero bytes there."""
-    w, h = 13, 11
+@pytest.mark.parametrize("w,h", [(13, 11), (14, 11), (1006, 757)])
+def test_packed_stream_bound_ignores_larger_buffer(bd, algo, w, h):
+    """A group that runs past the packed stream (pixel counts that do not
+    fill the last group) decodes as 0 (pixel_utils.hpp:69-98, bounded by
+    compute_packed_byte_size), even when the caller's buffer is longer and
+    holds non-zero bytes there.  14 x 11 and 1006 x 757 take the
+    register-resident kernel (w % 4 == 2) with a cut-short final 10-bit group."""
     rng = np.random.default_rng(bd)
     vals = rng.integers(0, 1 << bd, size=(h, w)).astype(np.uint16)
     raw = scene.raw_bytes(vals, bd, True)
