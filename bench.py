@@ -115,6 +115,165 @@ def cpu_baseline(raw, name, budget_s=12.0):
     }
 
 
+def bench_batch(steps, warmup, with_cpu):
+    """The mixed-size batch: 48 x 1006x758 + 16 x 4024x3032 MG frames with
+    seeded bit depths {8u, 10p, 12p} and black levels {0, 16, 64}, u16 RGB.
+    A step is the whole batch.  Times (a) the plain per-frame launches (one
+    imop_hip_demosaic per frame, captured into a HIP graph so that host
+    launch overhead is not counted; 8-bit frames write u8 there, so this
+    baseline moves fewer bytes than the batch), (b) the direct batched call and (c) the graph-captured batch plan with the
+    shape the autotuner picks, each over two rotating buffer sets."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    MG = imop_amd.DemosaicAlgorithm.MG
+    spec = scene.mixed_batch_spec(seed=4)
+    dev = torch.device("cuda:0")
+    log(f"bench: batch of {len(spec)} frames, generating scenes")
+    large_scene = {}
+    raws = []
+    for f in spec:
+        if f["w"] * f["h"] > 4_000_000:
+            key = (f["w"], f["h"], f["bd"])
+            if key not in large_scene:
+                large_scene[key] = scene.raw_bytes(scene.make_scene(f["h"], f["w"], f["bd"], 0, seed=f["bd"]),
+                                                   f["bd"], f["packed"])
+            raws.append(large_scene[key])
+        else:
+            raws.append(scene.raw_bytes(scene.make_scene(f["h"], f["w"], f["bd"], 0, seed=f["seed"]), f["bd"],
+                                        f["packed"]))
+    nsets = 2
+    d_in = [[torch.from_numpy(r.copy()).to(dev) for r in raws] for _ in range(nsets)]
+    d_out = [[torch.empty(f["w"] * f["h"] * 6, dtype=torch.uint8, device=dev) for f in spec] for _ in range(nsets)]
+    px = sum(f["w"] * f["h"] for f in spec)
+    in_bytes = sum(int(r.size) for r in raws)
+    alg_bytes = in_bytes + 6 * px
+    stream = torch.cuda.current_stream()
+
+    def per_frame(k):
+        for f, di, do in zip(spec, d_in[k % nsets], d_out[k % nsets]):
+            if imop_amd.demosaic(di, do, f["w"], f["h"], 0, MG, f["bd"], f["packed"], [f["bl"]] * 4) != 0:
+                raise RuntimeError("imop_hip_demosaic failed")
+
+    descs = [[imop_amd.frame_desc(di, do, f["w"], f["h"], 0, MG, f["bd"], f["packed"], [f["bl"]] * 4, False, 2)
+              for f, di, do in zip(spec, d_in[s], d_out[s])] for s in range(nsets)]
+    plans = [imop_amd.BatchPlan(descs[s]) for s in range(nsets)]
+    for p in plans:
+        if p.error != 0:
+            raise RuntimeError(f"batch plan failed: {p.error}")
+
+    # parity: every frame of the batch equals the single-frame path, by value
+    per_frame(0)
+    torch.cuda.synchronize()
+    ref = [do[: f["w"] * f["h"] * (6 if f["bd"] > 8 else 3)].clone() for f, do in zip(spec, d_out[0])]
+    plans[0].launch(stream)
+    torch.cuda.synchronize()
+    ok = True
+    for f, do, r in zip(spec, d_out[0], ref):
+        got = do.view(torch.int16).to(torch.int32) & 0xFFFF
+        want = r.to(torch.int32) if f["bd"] <= 8 else (r.view(torch.int16).to(torch.int32) & 0xFFFF)
+        ok = ok and bool(torch.equal(got, want))
+    del ref
+    parity = "every frame equals the single-frame path" if ok else "MISMATCH"
+    log(f"bench: batch parity {parity}")
+
+    # launch shape of the batched kernel: the project's autotuner over graph plans
+    def digest():
+        acc = 0
+        for k, do in enumerate(d_out[0]):
+            acc += int((do.view(torch.int32).to(torch.int64) * (2 * k + 1)).sum().item())
+        return acc
+
+    from imop_amd import autotune
+    log("bench: autotuning the batched kernel")
+    best = autotune.tune_batch(descs, digest=digest, verbose=False)
+    shape = dict(rows=best[0], waves_per_block=best[1], blocks_per_cu=best[2], order=best[3])
+    log(f"bench: batch shape {shape} ({best[4]:.4f} ms)")
+    for p in plans:
+        p.close()
+    plans = [imop_amd.BatchPlan(descs[s]) for s in range(nsets)]
+
+    graphs = []
+    for s in range(nsets):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            per_frame(s)
+        graphs.append(g)
+
+    def timed(fn):
+        for k in range(warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for k in range(steps):
+            fn(k)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    ms_frames_eager = timed(per_frame)
+    ms_frames = timed(lambda k: graphs[k % nsets].replay())
+    ms_direct = timed(lambda k: imop_amd.demosaic_batch(descs[k % nsets], stream))
+    ms_plan = timed(lambda k: plans[k % nsets].launch(stream))
+    for p in plans:
+        p.close()
+    mpx = lambda ms: round(px / (ms / 1e3) / 1e6, 1)
+    log(f"bench: batch per-frame {ms_frames:.4f} ms (eager loop {ms_frames_eager:.4f}), direct {ms_direct:.4f} ms, "
+        f"plan {ms_plan:.4f} ms")
+    out = {
+        "workload": "MG demosaic batch 48 x 1006x758 + 16 x 4024x3032, depths {8u,10p,12p}, black level {0,16,64}, "
+                    "u16 RGB (seeded)",
+        "frames": len(spec),
+        "pixels_per_step": px,
+        "value": mpx(ms_plan),
+        "unit": "Mpix/s",
+        "ms_per_step": round(ms_plan, 5),
+        "per_frame_launches_mpix_s": mpx(ms_frames),
+        "per_frame_launches_eager_mpix_s": mpx(ms_frames_eager),
+        "direct_batch_call_mpix_s": mpx(ms_direct),
+        "speedup_vs_per_frame": round(ms_frames / ms_plan, 3),
+        "batch_shape": shape,
+        "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (ms_plan / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg_bytes / (ms_plan / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_step": alg_bytes},
+        "parity": parity,
+        "steps": steps,
+        "warmup": warmup,
+    }
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(spec, raws)
+    return out
+
+
+def cpu_baseline_batch(spec, raws, budget_s=10.0):
+    """The reference CPU path over the batch's frames in order, for about
+    budget_s seconds: pixels done / time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if O.reference_available():
+        fn, kind, threads = O.ref_demosaic, "reference", O.reference().ref_num_threads()
+    else:
+        fn, kind, threads = O.demosaic, "port", O.oracle().orc_num_threads()
+    done_px, n = 0, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and n < 4 * len(spec):
+        f, raw = spec[n % len(spec)], raws[n % len(spec)]
+        e, _ = fn(raw, f["w"], f["h"], 0, 2, f["bd"], f["packed"], [f["bl"]] * 4)
+        if e != 0:
+            raise RuntimeError(f"CPU baseline returned {e}")
+        done_px += f["w"] * f["h"]
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(done_px / dt / 1e6, 2), "unit": "Mpix/s", "cores": int(threads), "kind": kind,
+            "sample": f"the first {n} frames of the batch in order ({done_px} px, {dt:.1f} s), one frame at a time"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +282,8 @@ def main():
     ap.add_argument("--workload", default="mg12", choices=sorted(WORKLOADS))
     ap.add_argument("--sets", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed loop, run 4 plain device copies of this many MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -240,6 +401,10 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(raw, args.workload)
+    if not args.no_batch and args.workload == "mg12":
+        del d_in, d_out
+        torch.cuda.empty_cache()
+        result["batched"] = bench_batch(args.batch_steps, 3, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 

@@ -1,0 +1,43 @@
+// host_common.hpp -- host-side helpers shared by the C ABI translation units:
+// argument checks with the reference's error codes, frame parameter setup,
+// per-device workspaces and the per-frame launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace imop_hip {
+
+// imop::DemosaicError (include/imop/types.hpp:31-39)
+enum : int { kOk = 0, kNull = 1, kDims = 2, kBitDepth = 3, kPattern = 4, kSmall = 5, kInternal = 6 };
+// imop::DemosaicAlgorithm (include/imop/types.hpp:19-29)
+enum : int { kSuperFast = 0, kHqli = 1, kMg = 2, kL7 = 3, kDfpd = 4, kAhd = 5, kAmaze = 6, kRcd = 7, kPrism = 8 };
+
+// pixel::compute_bayer_byte_size / compute_rgb_byte_size (pixel_utils.hpp:21-34)
+uint64_t bayer_bytes(int w, int h, int bd, bool packed);
+uint64_t rgb_bytes(int w, int h, int bd);
+
+// Argument checks in imop::demosaic_cuda's order (cuda_dispatch.cpp:70-90).
+int check_args(const void* in, const void* out, int w, int h, int pattern, int algo, int bd, int packed,
+               const float* bl, int bl_mode);
+
+// Black-level offsets by CFA parity (y & 1) * 2 + (x & 1), each max(0, .).
+void bl_by_parity(int pattern, const float* offs, int mode, float q[4]);
+
+// Kernel parameters of one frame; *mode = the HQLI / MG kernel mode (-1 for PRISM).
+FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int h, int pattern, int bd, int packed,
+                        const float* bl, int bl_mode, int algo, int* mode);
+
+// Per-device PRISM workspace of at least `need` bytes (grows, never shrinks).
+hipError_t get_scratch(size_t need, void** out);
+
+// Enqueue one frame: PRISM (workspace `prism_ws`, or the per-device one when
+// null), the register-resident kernel when it applies, else the LDS-tiled one.
+int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st);
+
+// Number of visible devices (probed once).
+int device_count_cached();
+
+}  // namespace imop_hip

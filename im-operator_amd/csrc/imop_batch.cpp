@@ -1,0 +1,317 @@
+// imop_batch.cpp -- batched demosaic over frames of mixed size, layout and
+// bit depth (imop_hip_demosaic_batch) and its graph-captured plan form.
+//
+// Frames the register-resident kernel takes (HQLI / MG, even width, u8 / u16
+// / 12-bit / 10-bit packed, integral black level, aligned buffers) are
+// grouped by (pattern, algorithm); each group becomes one device work list
+// and one persistent k_batch launch.  Every other frame gets its single-frame
+// launches on the same stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "batch.hpp"
+#include "host_common.hpp"
+#include "imop_hip.h"
+#include "plain.hpp"
+
+namespace imop_hip {
+
+hipError_t launch_batch_pattern0(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
+hipError_t launch_batch_pattern1(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
+hipError_t launch_batch_pattern2(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
+hipError_t launch_batch_pattern3(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
+
+hipError_t launch_batch(const BatchArgs& a, int pattern, int alg, int rows, int blocks, hipStream_t st) {
+    switch (pattern) {
+    case 0: return launch_batch_pattern0(a, alg, rows, blocks, st);
+    case 1: return launch_batch_pattern1(a, alg, rows, blocks, st);
+    case 2: return launch_batch_pattern2(a, alg, rows, blocks, st);
+    default: return launch_batch_pattern3(a, alg, rows, blocks, st);
+    }
+}
+
+namespace {
+
+// Launch shape of k_batch (imop_hip_set_batch_shape).
+std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
+
+int cu_count() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
+
+int out_elem_of(const imop_hip_frame_desc& d) {
+    return d.out_elem_size == 0 ? (d.bit_depth > 8 ? 2 : 1) : d.out_elem_size;
+}
+
+// imop_hip_demosaic's checks, per frame, plus the output element choice.
+int check_frame(const imop_hip_frame_desc& d) {
+    int e = check_args(d.in, d.out, d.width, d.height, d.pattern, d.algorithm, d.bit_depth, d.is_packed,
+                       d.black_level, d.black_level_mode);
+    if (e != kOk) return e;
+    const int elem = out_elem_of(d);
+    if (elem != 1 && elem != 2) return kInternal;
+    if (elem == 1 && d.bit_depth > 8) return kInternal;
+    const uint64_t px = (uint64_t)d.width * (uint64_t)d.height;
+    if (d.in_bytes < bayer_bytes(d.width, d.height, d.bit_depth, d.is_packed != 0) || d.out_bytes < px * 3u * elem)
+        return kInternal;
+    // u16 samples and u16 RGB are accessed as naturally aligned 16-bit words
+    if ((d.bit_depth > 8 && !d.is_packed && ((uintptr_t)d.in & 1)) || (elem == 2 && ((uintptr_t)d.out & 1)))
+        return kInternal;
+    return kOk;
+}
+
+struct Group {
+    int pattern = 0, alg = 0;
+    std::vector<BatchFrame> frames;
+    uint32_t total = 0;
+};
+
+struct Single {
+    FrameParams f;
+    int algo, mode;
+};
+
+struct Prepared {
+    std::vector<Group> groups;
+    std::vector<Single> singles;
+    size_t prism_ws = 0;
+    int rows = 4, wpb = 4, bpc = 0, order = 2;
+};
+
+int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
+    if (failed) *failed = -1;
+    if (n < 0) return kInternal;
+    if (n > 0 && !fr) return kNull;
+    for (int i = 0; i < n; ++i) {
+        const int e = check_frame(fr[i]);
+        if (e != kOk) {
+            if (failed) *failed = i;
+            return e;
+        }
+    }
+    P.rows = g_rows;
+    P.wpb = g_wpb;
+    P.bpc = g_bpc;
+    P.order = g_order;
+    std::map<std::pair<int, int>, size_t> index;
+    for (int i = 0; i < n; ++i) {
+        const imop_hip_frame_desc& d = fr[i];
+        int mode = -1;
+        FrameParams f = make_params(d.in, d.in_bytes, d.out, d.width, d.height, d.pattern, d.bit_depth, d.is_packed,
+                                    d.black_level, d.black_level_mode, d.algorithm, &mode);
+        f.out16 = out_elem_of(d) == 2;
+        const int nsx = (d.width + 247) / 248, nby = (d.height + P.rows - 1) / P.rows;
+        // k_batch runs MG 8-bit frames on the integer path; a packed 8-bit
+        // frame without black level takes the exact path, so it runs alone
+        const bool mode_ok = d.algorithm == kHqli || (d.algorithm == kMg && (f.fmt == FMT_U8) == (mode == MODE_MG_INT8));
+        const bool batched = (d.algorithm == kHqli || d.algorithm == kMg) && mode_ok && fast_path_ok(f) &&
+                             nsx <= 65535 && nby <= 65535;
+        if (!batched) {
+            P.singles.push_back(Single{f, d.algorithm, mode});
+            if (d.algorithm == kPrism) P.prism_ws = std::max(P.prism_ws, prism_workspace_bytes(d.width, d.height));
+            continue;
+        }
+        const auto key = std::make_pair(d.pattern, d.algorithm == kMg ? 1 : 0);
+        auto it = index.find(key);
+        if (it == index.end()) {
+            it = index.emplace(key, P.groups.size()).first;
+            P.groups.emplace_back();
+            P.groups.back().pattern = key.first;
+            P.groups.back().alg = key.second;
+        }
+        Group& g = P.groups[it->second];
+        const uint64_t items = (uint64_t)nsx * (uint64_t)nby;
+        if ((uint64_t)g.total + items >= (1ull << 31)) return kInternal;
+        BatchFrame b{};
+        b.in = f.in;
+        b.out = f.out;
+        b.in_bytes = (uint32_t)f.in_bytes;
+        b.w = f.w;
+        b.h = f.h;
+        b.bd = f.bd;
+        b.maxv = f.maxv;
+        b.clamp_hi = f.clamp_hi;
+        b.fmt = f.fmt;
+        b.flags = (f.out16 ? 1 : 0) | (f.bl_on ? 2 : 0);
+        for (int k = 0; k < 4; ++k) b.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
+        b.nsx = (uint16_t)nsx;
+        b.nby = (uint16_t)nby;
+        b.first = g.total;
+        g.frames.push_back(b);
+        g.total += (uint32_t)items;
+    }
+    return kOk;
+}
+
+// Enqueue every launch of a prepared batch; tables[i] = device copy of groups[i].frames.
+int enqueue(const Prepared& P, const std::vector<void*>& tables, void* prism_ws, hipStream_t st) {
+    const int cus = cu_count();
+    for (size_t i = 0; i < P.groups.size(); ++i) {
+        const Group& g = P.groups[i];
+        if (g.total == 0) continue;
+        BatchArgs a;
+        a.frames = (const BatchFrame*)tables[i];
+        a.nframes = (int)g.frames.size();
+        a.total = g.total;
+        a.wpb = P.wpb;
+        a.order = P.order;
+        int64_t blocks = ((int64_t)g.total + P.wpb - 1) / P.wpb;
+        if (P.bpc > 0 && cus > 0) blocks = std::min<int64_t>(blocks, (int64_t)P.bpc * cus);
+        if (launch_batch(a, g.pattern, g.alg, P.rows, (int)blocks, st) != hipSuccess) return kInternal;
+    }
+    for (const Single& s : P.singles) {
+        const int e = launch_frame(s.f, s.algo, s.mode, prism_ws, st);
+        if (e != kOk) return e;
+    }
+    return kOk;
+}
+
+}  // namespace
+
+}  // namespace imop_hip
+
+using namespace imop_hip;
+
+struct imop_hip_batch_plan {
+    int dev = 0;
+    std::vector<void*> tables;
+    void* prism_ws = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void free_plan(imop_hip_batch_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (void* t : p->tables)
+        if (t) (void)hipFree(t);
+    if (p->prism_ws) (void)hipFree(p->prism_ws);
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stream, int* failed_frame) {
+    if (failed_frame) *failed_frame = -1;
+    if (device_count_cached() == 0) return kInternal;
+    Prepared P;
+    int e = prepare(frames, n, P, failed_frame);
+    if (e != kOk) return e;
+    hipStream_t st = (hipStream_t)stream;
+    // work lists live in stream-ordered allocations released after the launch
+    std::vector<void*> tables(P.groups.size(), nullptr);
+    for (size_t i = 0; i < P.groups.size(); ++i) {
+        const size_t bytes = P.groups[i].frames.size() * sizeof(BatchFrame);
+        if (hipMallocAsync(&tables[i], bytes, st) != hipSuccess ||
+            hipMemcpyAsync(tables[i], P.groups[i].frames.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+            e = kInternal;
+            break;
+        }
+    }
+    if (e == kOk) e = enqueue(P, tables, nullptr, st);
+    for (void* t : tables)
+        if (t) (void)hipFreeAsync(t, st);
+    return e;
+}
+
+int imop_hip_batch_plan_create(const imop_hip_frame_desc* frames, int n, imop_hip_batch_plan** plan,
+                               int* failed_frame) {
+    if (failed_frame) *failed_frame = -1;
+    if (!plan) return kNull;
+    *plan = nullptr;
+    if (device_count_cached() == 0) return kInternal;
+    Prepared P;
+    int e = prepare(frames, n, P, failed_frame);
+    if (e != kOk) return e;
+    imop_hip_batch_plan* p = new imop_hip_batch_plan();
+    if (hipGetDevice(&p->dev) != hipSuccess) {
+        free_plan(p);
+        return kInternal;
+    }
+    p->tables.assign(P.groups.size(), nullptr);
+    for (size_t i = 0; i < P.groups.size(); ++i) {
+        const size_t bytes = P.groups[i].frames.size() * sizeof(BatchFrame);
+        if (hipMalloc(&p->tables[i], bytes) != hipSuccess ||
+            hipMemcpy(p->tables[i], P.groups[i].frames.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            free_plan(p);
+            return kInternal;
+        }
+    }
+    if (P.prism_ws && hipMalloc(&p->prism_ws, P.prism_ws) != hipSuccess) {
+        free_plan(p);
+        return kInternal;
+    }
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        free_plan(p);
+        return kInternal;
+    }
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        free_plan(p);
+        return kInternal;
+    }
+    e = enqueue(P, p->tables, p->prism_ws, cs);
+    const hipError_t ce = hipStreamEndCapture(cs, &p->graph);
+    (void)hipStreamDestroy(cs);
+    if (e != kOk || ce != hipSuccess ||
+        hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
+        free_plan(p);
+        return kInternal;
+    }
+    *plan = p;
+    return kOk;
+}
+
+int imop_hip_batch_plan_launch(imop_hip_batch_plan* plan, void* stream) {
+    if (!plan) return kNull;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev != plan->dev) return kInternal;
+    return hipGraphLaunch(plan->exec, (hipStream_t)stream) == hipSuccess ? kOk : kInternal;
+}
+
+int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan) {
+    if (!plan) return kNull;
+    free_plan(plan);
+    return kOk;
+}
+
+int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order) {
+    if (rows_per_wave == 4 || rows_per_wave == 8) g_rows = rows_per_wave;
+    if (waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4) g_wpb = waves_per_block;
+    if (blocks_per_cu >= 0 && blocks_per_cu <= 64) g_bpc = blocks_per_cu;
+    if (order >= 0 && order <= 3) g_order = order & 2;
+    return kOk;
+}
+
+int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order) {
+    if (rows_per_wave) *rows_per_wave = g_rows;
+    if (waves_per_block) *waves_per_block = g_wpb;
+    if (blocks_per_cu) *blocks_per_cu = g_bpc;
+    if (order) *order = g_order;
+    return kOk;
+}
+
+}  // extern "C"

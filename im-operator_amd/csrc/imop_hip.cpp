@@ -7,17 +7,11 @@
 #include <mutex>
 #include <vector>
 
+#include "host_common.hpp"
 #include "imop_hip.h"
 #include "plain.hpp"
 
-using namespace imop_hip;
-
-namespace {
-
-// imop::DemosaicError (include/imop/types.hpp:31-39)
-enum : int { kOk = 0, kNull = 1, kDims = 2, kBitDepth = 3, kPattern = 4, kSmall = 5, kInternal = 6 };
-// imop::DemosaicAlgorithm (include/imop/types.hpp:19-29)
-enum : int { kSuperFast = 0, kHqli = 1, kMg = 2, kL7 = 3, kDfpd = 4, kAhd = 5, kAmaze = 6, kRcd = 7, kPrism = 8 };
+namespace imop_hip {
 
 // Minimum frame side per algorithm, as imop::demosaic_cuda checks it
 // (src/algorithms/cuda_dispatch.cpp:77-89).
@@ -143,6 +137,23 @@ hipError_t get_scratch(size_t need, void** out) {
     return hipSuccess;
 }
 
+int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st) {
+    hipError_t he;
+    if (algo == kPrism) {
+        void* ws = prism_ws;
+        if (!ws) {
+            he = get_scratch(prism_workspace_bytes(f.w, f.h), &ws);
+            if (he != hipSuccess) return kInternal;
+        }
+        he = launch_prism(f, ws, st);
+    } else if (fast_path_ok(f)) {
+        he = launch_fast(f, mode, st);
+    } else {
+        he = launch_plain(f, mode, st);
+    }
+    return he == hipSuccess ? kOk : kInternal;
+}
+
 int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int w, int h, int pattern,
                  int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st) {
     int e = check_args(d_in, d_out, w, h, pattern, algo, bd, packed, bl, bl_mode);
@@ -152,19 +163,10 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
     if (bd > 8 && ((!packed && ((uintptr_t)d_in & 1)) || ((uintptr_t)d_out & 1))) return kInternal;
     int mode = -1;
     FrameParams f = make_params(d_in, in_bytes, d_out, w, h, pattern, bd, packed, bl, bl_mode, algo, &mode);
-    hipError_t he;
-    if (algo == kPrism) {
-        void* ws = nullptr;
-        he = get_scratch(prism_workspace_bytes(w, h), &ws);
-        if (he != hipSuccess) return kInternal;
-        he = launch_prism(f, ws, st);
-    } else if (fast_path_ok(f)) {
-        he = launch_fast(f, mode, st);
-    } else {
-        he = launch_plain(f, mode, st);
-    }
-    return he == hipSuccess ? kOk : kInternal;
+    return launch_frame(f, algo, mode, nullptr, st);
 }
+
+namespace {
 
 // Per-host-thread device buffers for the host-pointer entry points, like the
 // reference's thread_local CudaWorkspace (cuda_dispatch.cpp:51).
@@ -207,6 +209,8 @@ struct HostStaging {
 };
 thread_local HostStaging t_staging;
 
+}  // namespace
+
 int device_count_cached() {
     static std::once_flag once;
     static int count = 0;
@@ -217,7 +221,9 @@ int device_count_cached() {
     return count;
 }
 
-}  // namespace
+}  // namespace imop_hip
+
+using namespace imop_hip;
 
 extern "C" {
 

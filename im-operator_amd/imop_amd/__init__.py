@@ -7,6 +7,8 @@ demosaic path:
   * demosaic(...)          imop::demosaic_cuda with device tensors (stream-ordered)
   * demosaic_host(...)     imop::demosaic_cuda with host arrays (synchronous)
   * black_level(...)       black_level::process_black_level_cuda
+  * demosaic_batch(...)    frames of mixed size / depth in one persistent launch
+  * BatchPlan(...)         the same batch captured into a HIP graph
 
 Names, enumerations and error codes follow the reference
 (include/imop/types.hpp:12-39, include/imop/algorithms.hpp:19-84).  If the
@@ -67,6 +69,25 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+class FrameDesc(ctypes.Structure):
+    """imop_hip_frame_desc (include/imop_hip.h)."""
+    _fields_ = [
+        ("in_", ctypes.c_void_p),
+        ("in_bytes", ctypes.c_uint64),
+        ("out", ctypes.c_void_p),
+        ("out_bytes", ctypes.c_uint64),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("pattern", ctypes.c_int32),
+        ("algorithm", ctypes.c_int32),
+        ("bit_depth", ctypes.c_int32),
+        ("is_packed", ctypes.c_int32),
+        ("out_elem_size", ctypes.c_int32),
+        ("black_level_mode", ctypes.c_int32),
+        ("black_level", ctypes.c_float * 4),
+    ]
+
+
 _lib = None
 
 
@@ -94,6 +115,18 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_set_launch_shape.restype = i
         L.imop_hip_get_launch_shape.argtypes = [ctypes.POINTER(i)] * 3
         L.imop_hip_get_launch_shape.restype = i
+        L.imop_hip_demosaic_batch.argtypes = [ctypes.POINTER(FrameDesc), i, vp, ctypes.POINTER(i)]
+        L.imop_hip_demosaic_batch.restype = i
+        L.imop_hip_batch_plan_create.argtypes = [ctypes.POINTER(FrameDesc), i, ctypes.POINTER(vp), ctypes.POINTER(i)]
+        L.imop_hip_batch_plan_create.restype = i
+        L.imop_hip_batch_plan_launch.argtypes = [vp, vp]
+        L.imop_hip_batch_plan_launch.restype = i
+        L.imop_hip_batch_plan_destroy.argtypes = [vp]
+        L.imop_hip_batch_plan_destroy.restype = i
+        L.imop_hip_set_batch_shape.argtypes = [i, i, i, i]
+        L.imop_hip_set_batch_shape.restype = i
+        L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
+        L.imop_hip_get_batch_shape.restype = i
         _lib = L
     return _lib
 
@@ -182,6 +215,98 @@ def black_level(d_data, width: int, height: int, pattern: int, algorithm: int, b
     return lib().imop_hip_black_level(ctypes.c_void_p(d_data.data_ptr()), width, height, int(pattern),
                                       int(algorithm), bit_depth, o.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                       ctypes.c_void_p(stream))
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return stream
+
+
+def frame_desc(d_in, d_out, width: int, height: int, pattern: int, algorithm: int, bit_depth: int,
+               is_packed: bool = False, black_level=None, per_channel: bool = False, out_elem_size: int = 0):
+    """One imop_hip_frame_desc over device tensors (the tensors must outlive its use)."""
+    d = FrameDesc()
+    d.in_ = d_in.data_ptr()
+    d.in_bytes = d_in.numel() * d_in.element_size()
+    d.out = d_out.data_ptr()
+    d.out_bytes = d_out.numel() * d_out.element_size()
+    d.width, d.height = width, height
+    d.pattern, d.algorithm, d.bit_depth = int(pattern), int(algorithm), bit_depth
+    d.is_packed = int(bool(is_packed))
+    d.out_elem_size = out_elem_size
+    bl, mode = _bl_args(black_level, per_channel)
+    d.black_level_mode = mode
+    if bl is not None:
+        for k in range(4):
+            d.black_level[k] = float(bl[k])
+    return d
+
+
+def _desc_array(descs):
+    arr = (FrameDesc * max(1, len(descs)))()
+    for k, d in enumerate(descs):
+        arr[k] = d
+    return arr
+
+
+def demosaic_batch(descs, stream=None) -> tuple[int, int]:
+    """imop_hip_demosaic_batch over a list of FrameDesc: (DemosaicError, failed frame or -1)."""
+    arr = _desc_array(descs)
+    failed = ctypes.c_int(-1)
+    e = lib().imop_hip_demosaic_batch(arr, len(descs), ctypes.c_void_p(_stream_handle(stream)),
+                                      ctypes.byref(failed))
+    return e, failed.value
+
+
+class BatchPlan:
+    """A batch captured into a HIP graph (imop_hip_batch_plan_*)."""
+
+    def __init__(self, descs):
+        self._h = ctypes.c_void_p()
+        failed = ctypes.c_int(-1)
+        self.error = lib().imop_hip_batch_plan_create(_desc_array(descs), len(descs), ctypes.byref(self._h),
+                                                      ctypes.byref(failed))
+        self.failed_frame = failed.value
+        if self.error != 0:
+            self._h = None
+
+    def launch(self, stream=None) -> int:
+        if self._h is None:
+            raise RuntimeError(f"batch plan was not created (error {self.error})")
+        return lib().imop_hip_batch_plan_launch(self._h, ctypes.c_void_p(_stream_handle(stream)))
+
+    def close(self) -> None:
+        if self._h is not None:
+            lib().imop_hip_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_batch_shape(rows_per_wave: int = 0, waves_per_block: int = 0, blocks_per_cu: int = -1,
+                    order: int = -1) -> None:
+    """Launch shape of the batched kernel (see imop_hip.h)."""
+    lib().imop_hip_set_batch_shape(rows_per_wave, waves_per_block, blocks_per_cu, order)
+
+
+def get_batch_shape() -> tuple[int, int, int, int]:
+    v = [ctypes.c_int() for _ in range(4)]
+    lib().imop_hip_get_batch_shape(*[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def set_launch_shape(rows_per_wave: int = 0, waves_per_block: int = 0, xcd_remap: int = -1) -> None:

@@ -83,6 +83,55 @@ def tune(w: int, h: int, bd: int, packed: bool, algo: int, black_level=None, see
     return best[0], best[1], best[2], best[3], table
 
 
+BATCH_CANDIDATES = list(itertools.product((4, 8), (1, 2, 4), (0, 2, 4, 8), (0, 2)))
+
+
+def tune_batch(desc_sets, digest=None, candidates=BATCH_CANDIDATES, iters: int = 8, verbose: bool = True):
+    """Time every batched-kernel shape (rows, waves per workgroup, workgroups
+    per CU, order) on graph plans over the given frame-descriptor sets (one
+    list of imop_amd.FrameDesc per rotating buffer set, whose tensors the
+    caller keeps alive).  `digest()` (optional) returns a checksum of the
+    first set's outputs; every shape must give the same one.  The fastest
+    shape is applied and returned as (rows, wpb, bpc, order, ms_per_batch, table)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    table = []
+    ref = None
+    for shape in candidates:
+        imop_amd.set_batch_shape(*shape)
+        plans = [imop_amd.BatchPlan(d) for d in desc_sets]
+        for p in plans:
+            if p.error != 0:
+                raise RuntimeError(f"batch plan failed ({p.error}) for shape {shape}")
+        if digest is not None:
+            plans[0].launch(stream)
+            torch.cuda.synchronize()
+            got = digest()
+            if ref is None:
+                ref = got
+            elif got != ref:
+                raise RuntimeError(f"batch shape {shape} changed the output")
+        for k in range(2):
+            plans[k % len(plans)].launch(stream)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for k in range(iters):
+            plans[k % len(plans)].launch(stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / iters
+        for p in plans:
+            p.close()
+        table.append((*shape, ms))
+        if verbose:
+            print(f"autotune batch rows={shape[0]} wpb={shape[1]} bpc={shape[2]} order={shape[3]}: {ms:.4f} ms",
+                  flush=True)
+    best = min(table, key=lambda t: t[4])
+    imop_amd.set_batch_shape(*best[:4])
+    return (*best, table)
+
+
 def load_cache(path: str) -> dict:
     try:
         with open(path) as f:

@@ -157,3 +157,21 @@ def raw_bytes(values: np.ndarray, bd: int, packed: bool) -> np.ndarray:
     if bd <= 8:
         return np.ascontiguousarray(values, dtype=np.uint8).ravel()
     return np.ascontiguousarray(values, dtype="<u2").view(np.uint8).ravel()
+
+
+def mixed_batch_spec(seed: int = 4, n_small: int = 48, n_large: int = 16, small=(1006, 758),
+                     large=(4024, 3032)) -> list[dict]:
+    """Frames of the mixed-size batch workload: n_small frames of `small` and
+    n_large of `large` (w, h), each with a bit depth drawn from {8-bit
+    unpacked, 10-bit packed, 12-bit packed} and a global black level from
+    {0, 16, 64}, in a seeded order.  Entries: dict(w, h, bd, packed, bl, seed)."""
+    rng = np.random.default_rng(seed)
+    depths = [(8, False), (10, True), (12, True)]
+    frames = []
+    for k in range(n_small + n_large):
+        w, h = small if k < n_small else large
+        bd, packed = depths[int(rng.integers(0, 3))]
+        bl = [0.0, 16.0, 64.0][int(rng.integers(0, 3))]
+        frames.append(dict(w=w, h=h, bd=bd, packed=packed, bl=bl, seed=seed * 1000 + k))
+    order = rng.permutation(len(frames))
+    return [frames[i] for i in order]

@@ -98,6 +98,61 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
 int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
 int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap);
 
+/* ---------------------------------------------------------------- batches */
+
+/* One frame of a batch.  Buffers are device memory.  out_elem_size 0 keeps
+ * demosaic_cuda's output element (u8 for bit_depth <= 8, else u16); 2 writes
+ * u16 for every depth (8-bit values widened, same numbers); 1 is accepted for
+ * bit_depth <= 8 only.  black_level / black_level_mode as in
+ * imop_hip_demosaic.  pattern / algorithm may differ between frames. */
+typedef struct imop_hip_frame_desc {
+    const void* in;
+    uint64_t in_bytes;
+    void* out;
+    uint64_t out_bytes;
+    int32_t width, height;
+    int32_t pattern;
+    int32_t algorithm;
+    int32_t bit_depth;
+    int32_t is_packed;
+    int32_t out_elem_size;
+    int32_t black_level_mode;
+    float black_level[4];
+} imop_hip_frame_desc;
+
+/* Stream-ordered demosaic of n frames of any size, layout, bit depth and
+ * black level.  HQLI / MG frames of even width with u8 / u16 / 12-bit /
+ * 10-bit packed input run in one persistent launch per (pattern, algorithm)
+ * (the register-resident kernel walking a work list of every frame's bands);
+ * other frames (PRISM, odd widths, other packed depths) get their
+ * single-frame launches on the same stream.  Every frame is checked first,
+ * as imop_hip_demosaic checks it; on the first invalid frame nothing is
+ * enqueued, its index goes to *failed_frame (may be NULL) and its error code
+ * is returned.  Replaces imop::demosaic_cuda_batch
+ * (src/algorithms/cuda_dispatch.cpp:150-209: one size, one depth, a launch
+ * and two copies per frame). */
+int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stream, int* failed_frame);
+
+/* The same batch captured once into a HIP graph, for repeated launches over
+ * the same buffers (replaces cuda_graph_capture, src/algorithms/
+ * cuda_kernels.cu:1520-1688, which captured one frame size).  The plan owns
+ * the device work list, a PRISM workspace when needed, and the graph; the
+ * launch shape current at creation is used.  Launches on one plan are
+ * stream-ordered by the caller; the buffers must stay valid until destroy. */
+typedef struct imop_hip_batch_plan imop_hip_batch_plan;
+int imop_hip_batch_plan_create(const imop_hip_frame_desc* frames, int n, imop_hip_batch_plan** plan,
+                               int* failed_frame);
+int imop_hip_batch_plan_launch(imop_hip_batch_plan* plan, void* stream);
+int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan);
+
+/* Launch shape of the batched kernel, chosen by the autotuner: rows per band
+ * (4 or 8), waves per workgroup (1, 2 or 4), workgroups per CU of the
+ * persistent grid (1..64; 0 = one wave per work item), and wave order
+ * (bit 1: strip-fastest within a frame).  Out-of-range values leave a setting
+ * unchanged.  Process-wide; no counterpart in the reference. */
+int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order);
+int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order);
+
 #ifdef __cplusplus
 }
 #endif

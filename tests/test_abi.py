@@ -71,6 +71,27 @@ def test_no_device_behaves_like_reference_stub():
     e, _ = imop_amd.demosaic_host(raw, 8, 8, 0, imop_amd.DemosaicAlgorithm.MG, 8)
     assert e == imop_amd.DemosaicError.InternalError
     assert lib.imop_hip_synchronize() == 6
+    assert imop_amd.demosaic_batch([], stream=0) == (6, -1)
+    plan = imop_amd.BatchPlan([])
+    assert plan.error == 6
+
+
+def test_frame_desc_layout_matches_header(tmp_path):
+    """The ctypes mirror of imop_hip_frame_desc has the C compiler's layout."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    fields = [f[0] for f in imop_amd.FrameDesc._fields_]
+    cnames = ["in" if f == "in_" else f for f in fields]
+    body = "".join(f'printf("%zu\\n", offsetof(imop_hip_frame_desc, {c}));' for c in cnames)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "imop_hip.h"\n'
+                   'int main(void) { printf("%zu\\n", sizeof(imop_hip_frame_desc)); ' + body + ' return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", HEADER_DIR, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(imop_amd.FrameDesc)] + [getattr(imop_amd.FrameDesc, f).offset for f in fields]
+    assert got == want
 
 
 def test_enums_match_reference_numbering():
