@@ -58,6 +58,17 @@ def oracle() -> ctypes.CDLL:
         lib.orc_packed_bytes.argtypes = [ctypes.c_int] * 3
         lib.orc_set_mg8_scalar.argtypes = [ctypes.c_int]
         lib.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _dp = ctypes.POINTER(ctypes.c_double)
+        _lp = ctypes.POINTER(ctypes.c_long)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        lib.orc_bilinear_csr.restype = ctypes.c_long
+        lib.orc_bilinear_csr.argtypes = [ctypes.c_int] * 3 + [_lp, _ip, _dp]
+        lib.orc_chain_fwd64.argtypes = [_fp] + [ctypes.c_int] * 4 + [ctypes.c_double, _dp, _dp, ctypes.c_double,
+                                                                      _dp, _fp, _dp]
+        lib.orc_chain_vjp64.argtypes = [_fp] + [ctypes.c_int] * 4 + [ctypes.c_double, _dp, _dp, ctypes.c_double,
+                                                                      _fp, _fp, _dp, _dp, _dp]
+        lib.orc_bilinear_apply64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
+        lib.orc_bilinear_apply_t64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
         _oracle = lib
     return _oracle
 
@@ -149,3 +160,84 @@ def unpack(raw: np.ndarray, w: int, h: int, bd: int, packed: bool) -> np.ndarray
     out = np.zeros(w * h * (2 if bd > 8 else 1), dtype=np.uint8)
     lib.orc_unpack(_ptr(raw), w, h, bd, int(packed), _ptr(out))
     return (out.view(np.uint16) if bd > 8 else out).reshape(h, w)
+
+
+# ------------------------------------------------------------ FP64 ISP chain
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def bilinear_csr(w: int, h: int, pattern: int):
+    """The assembled bilinear demosaic operator D (scipy CSR, FP64): rows =
+    3 * w * h output samples (pixel-major), columns = w * h raw pixels."""
+    import scipy.sparse as sp
+    L = oracle()
+    nnz = L.orc_bilinear_csr(w, h, pattern, None, None, None)
+    rowptr = np.zeros(w * h * 3 + 1, dtype=np.int64)
+    col = np.zeros(max(nnz, 1), dtype=np.int32)
+    val = np.zeros(max(nnz, 1), dtype=np.float64)
+    L.orc_bilinear_csr(w, h, pattern, rowptr.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                       col.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(val))
+    return sp.csr_matrix((val[:nnz], col[:nnz], rowptr), shape=(w * h * 3, w * h))
+
+
+def bilinear_apply64(x: np.ndarray, pattern: int) -> np.ndarray:
+    """D x for one frame x[h, w] (FP64) -> [h, w, 3]."""
+    h, w = x.shape
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(h * w * 3, dtype=np.float64)
+    oracle().orc_bilinear_apply64(_d(x), w, h, pattern, _d(out))
+    return out.reshape(h, w, 3)
+
+
+def bilinear_apply_t64(v: np.ndarray, pattern: int) -> np.ndarray:
+    """D^T v for one frame v[h, w, 3] (FP64) -> [h, w]."""
+    h, w, _ = v.shape
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(h * w, dtype=np.float64)
+    oracle().orc_bilinear_apply_t64(_d(v), w, h, pattern, _d(out))
+    return out.reshape(h, w)
+
+
+def chain_fwd64(raw: np.ndarray, pattern: int, max_value: float, gains, ccm, gamma: float, target=None):
+    """FP64 chain forward over raw[n, h, w] (float32, units of max_value):
+    returns (out[n, h, w, 3] FP64, MSE vs target or None)."""
+    n, h, w = raw.shape
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    g = np.ascontiguousarray(gains, dtype=np.float64)
+    M = np.ascontiguousarray(ccm, dtype=np.float64).ravel()
+    out = np.zeros((n, h, w, 3), dtype=np.float64)
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
+    loss = ctypes.c_double(0.0)
+    rc = oracle().orc_chain_fwd64(_f(raw), n, w, h, pattern, float(max_value), _d(g), _d(M), 1.0 / gamma,
+                                  _d(out), _f(t), ctypes.byref(loss))
+    assert rc == 0
+    return out, (loss.value if target is not None else None)
+
+
+def chain_vjp64(raw: np.ndarray, pattern: int, max_value: float, gains, ccm, gamma: float, grad_out=None,
+                target=None):
+    """FP64 VJP of the chain (transposed assembled operator): returns
+    (grad_raw[n, h, w], grad_gains[3], grad_ccm[3, 3]); cotangent grad_out or
+    the MSE gradient against target."""
+    n, h, w = raw.shape
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    g = np.ascontiguousarray(gains, dtype=np.float64)
+    M = np.ascontiguousarray(ccm, dtype=np.float64).ravel()
+    go = None if grad_out is None else np.ascontiguousarray(grad_out, dtype=np.float32)
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
+    gr = np.zeros((n, h, w), dtype=np.float64)
+    gg = np.zeros(3, dtype=np.float64)
+    gm = np.zeros(9, dtype=np.float64)
+    rc = oracle().orc_chain_vjp64(_f(raw), n, w, h, pattern, float(max_value), _d(g), _d(M), 1.0 / gamma, _f(go),
+                                  _f(t), _d(gr), _d(gg), _d(gm))
+    assert rc == 0
+    return gr, gg, gm.reshape(3, 3)
