@@ -153,6 +153,42 @@ int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan);
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order);
 int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order);
 
+/* ---------------------------------------------- differentiable ISP chain */
+
+/* FP32 chain over n frames of one size, all buffers device memory:
+ *   x = raw / max_value, a = D x (bilinear demosaic: hqli.cpp:7-44 with exact
+ *   averages, fill_rgb_borders of common.hpp:47-96, corners 0),
+ *   u = clamp01(gains .* a)  (white balance, gpu_ops_kernels.cu:119-125),
+ *   z = clamp01(ccm u)       (CCM, gpu_ops_kernels.cu:92-96),
+ *   out = z^(1/gamma)        (gamma, gpu_ops_kernels.cu:57-64).
+ * raw [n][h][w]; out, target, grad_out [n][h][w][3]; gains[3] and ccm[9]
+ * (row-major) are device pointers.  The pair follows the
+ * torch.autograd.Function contract: forward keeps nothing; backward
+ * recomputes the forward from raw and the parameters.  No counterpart in the
+ * reference (its chain, gpu_ops_kernels.cu:40-135, has no backward).
+ *
+ * Forward: writes out (may be NULL) and, with target, the mean squared
+ * error to *d_loss (one device float). */
+int imop_hip_isp_chain_fwd(const float* d_raw, int n, int width, int height, int pattern, float max_value,
+                           const float* d_gains, const float* d_ccm, float gamma, float* d_out,
+                           const float* d_target, float* d_loss, void* stream);
+
+/* Backward (vector-Jacobian product): the cotangent is d_grad_out, or, when
+ * it is NULL, the gradient of the forward's mean squared error against
+ * d_target.  Writes dL/draw to d_grad_raw (may be NULL) and the 12 parameter
+ * gradients (gains[3] then ccm[9]) to d_grad_params (may be NULL), summed
+ * over frames.  Clamps pass the gradient where 0 <= v <= 1; the gamma
+ * derivative is 0 at 0.  Chain calls on one device share a partial-sum
+ * buffer: run them in stream order. */
+int imop_hip_isp_chain_bwd(const float* d_raw, int n, int width, int height, int pattern, float max_value,
+                           const float* d_gains, const float* d_ccm, float gamma, const float* d_grad_out,
+                           const float* d_target, float* d_grad_raw, float* d_grad_params, void* stream);
+
+/* The chain's demosaic operator alone, FP32, one frame: out[h][w][3] = D x
+ * and out[h][w] = D^T v (for checks against the transposed operator). */
+int imop_hip_bilinear_fwd(const float* d_x, int width, int height, int pattern, float* d_out, void* stream);
+int imop_hip_bilinear_vjp(const float* d_v, int width, int height, int pattern, float* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
