@@ -19,6 +19,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import hashlib
 import json
 import os
@@ -33,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (w, h, bd, packed, algo, black level, seed, diagonal lines)
     "mg12": (4024, 3032, 12, True, 2, 64.0, 2, 0),
-    "prism10": (4024, 3032, 10, True, 8, 64.0, 3, 200),
+    "prism10": (4024, 3032, 10, True, 8, 64.0, 3, 200),  # PRISM, 200 diagonal lines
     "hqli8": (1006, 758, 8, False, 1, None, 1, 0),
     "hqli12": (4024, 3032, 12, True, 1, 64.0, 2, 0),
 }
@@ -115,7 +116,7 @@ def cpu_baseline(raw, name, budget_s=12.0):
     }
 
 
-def bench_batch(steps, warmup, with_cpu):
+def bench_batch(steps, warmup, with_cpu, shape=None):
     """The mixed-size batch: 48 x 1006x758 + 16 x 4024x3032 MG frames with
     seeded bit depths {8u, 10p, 12p} and black levels {0, 16, 64}, u16 RGB.
     A step is the whole batch.  Times (a) the plain per-frame launches (one
@@ -188,10 +189,14 @@ def bench_batch(steps, warmup, with_cpu):
         return acc
 
     from imop_amd import autotune
-    log("bench: autotuning the batched kernel")
-    best = autotune.tune_batch(descs, digest=digest, verbose=False)
-    shape = dict(rows=best[0], waves_per_block=best[1], blocks_per_cu=best[2], order=best[3])
-    log(f"bench: batch shape {shape} ({best[4]:.4f} ms)")
+    if shape is None:
+        log("bench: autotuning the batched kernel")
+        best = autotune.tune_batch(descs, digest=digest, verbose=False)
+        shape = dict(rows=best[0], waves_per_block=best[1], blocks_per_cu=best[2], order=best[3])
+        log(f"bench: batch shape {shape} ({best[4]:.4f} ms)")
+    else:
+        imop_amd.set_batch_shape(*shape)
+        shape = dict(rows=shape[0], waves_per_block=shape[1], blocks_per_cu=shape[2], order=shape[3])
     for p in plans:
         p.close()
     plans = [imop_amd.BatchPlan(descs[s]) for s in range(nsets)]
@@ -250,6 +255,172 @@ def bench_batch(steps, warmup, with_cpu):
     return out
 
 
+def bench_single(name, steps, warmup, sets=4):
+    """One single-frame workload of WORKLOADS as a secondary measurement:
+    parity against the reference digest, then back-to-back launches over
+    rotating buffer sets."""
+    import hashlib as _h
+
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[name]
+    blv = None if bl is None else [bl] * 4
+    raw = scene.raw_bytes(scene.make_scene(h, w, bd, 0, seed=seed, n_lines=lines), bd, packed)
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(sets)]
+    d_out = [torch.empty(imop_amd.rgb_byte_size(w, h, bd), dtype=torch.uint8, device=dev) for _ in range(sets)]
+    stream = torch.cuda.current_stream()
+
+    def launch(k):
+        if imop_amd.demosaic(d_in[k % sets], d_out[k % sets], w, h, 0, algo, bd, packed, blv, stream=stream) != 0:
+            raise RuntimeError("imop_hip_demosaic failed")
+
+    launch(0)
+    torch.cuda.synchronize()
+    want = golden_sha(name)
+    got = _h.sha256(d_out[0].cpu().numpy().tobytes()).hexdigest()
+    parity = "unchecked" if want is None else ("bit-exact vs reference digest" if got == want else "MISMATCH")
+    for k in range(warmup):
+        launch(k)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for k in range(steps):
+        launch(k)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    alg = int(raw.size) + imop_amd.rgb_byte_size(w, h, bd)
+    log(f"bench: {name} {ms * 1e3:.1f} us per frame, parity {parity}")
+    return {"workload": name, "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+            "ms_per_frame": round(ms, 5), "parity": parity,
+            "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
+                         "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+
+
+def calibration_copies(mb):
+    """4 plain device copies of mb MiB: the known bytes rocprofv3's
+    FETCH_SIZE / WRITE_SIZE are scaled against."""
+    if mb <= 0:
+        return
+    import torch
+    n = mb << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"bench: 4 calibration copies of {n} bytes done")
+
+
+def bench_chain(steps, warmup, with_cpu):
+    """The differentiable chain on 8 x 1006x758 12-bit scene frames (FP32):
+    fused forward + MSE and the fused backward (VJP recomputing the forward),
+    back to back on the stream; then 200 Adam steps recovering seeded gains
+    and CCM from a rendered target."""
+    import numpy as np
+    import torch
+
+    from imop_amd import chain as C
+    from imop_amd import scene
+
+    n, w, h = 8, 1006, 758
+    dev = torch.device("cuda:0")
+    raw = torch.from_numpy(np.stack([scene.make_scene(h, w, 12, 0, seed=500 + k) for k in range(n)])
+                           .astype(np.float32)).to(dev)
+    rng = np.random.default_rng(2026)
+    g_t = rng.uniform(1.0, 2.5, 3)
+    m_t = np.eye(3) + rng.normal(0.0, 0.1, (3, 3))
+    gt = torch.tensor(g_t, dtype=torch.float32, device=dev)
+    mt = torch.tensor(m_t.ravel(), dtype=torch.float32, device=dev)
+    target, _ = C.chain_forward(raw, gt, mt, 0, 4095.0, 2.2)
+    g0 = torch.full((3,), 1.5, device=dev)
+    m0 = torch.eye(3, device=dev).reshape(9).contiguous()
+    stream = torch.cuda.current_stream()
+    # three rotating (raw, target, dL/draw) sets, 366 MB in all, so the timed
+    # kernels read and write HBM rather than the 256 MB Infinity Cache
+    raws = [raw] + [raw.clone() for _ in range(2)]
+    targets = [target] + [target.clone() for _ in range(2)]
+    grads = [torch.empty_like(raw) for _ in range(3)]
+    gp = torch.empty(12, dtype=torch.float32, device=dev)
+    from imop_amd.chain import _bind, _p
+    L = _bind()
+    hs = lambda: ctypes.c_void_p(stream.cuda_stream)
+    loss_buf = torch.empty(1, dtype=torch.float32, device=dev)
+
+    def fwd(k):
+        L.imop_hip_isp_chain_fwd(_p(raws[k % 3]), n, w, h, 0, 4095.0, _p(g0), _p(m0), 2.2, None, _p(targets[k % 3]),
+                                 _p(loss_buf), hs())
+
+    def bwd(k):
+        L.imop_hip_isp_chain_bwd(_p(raws[k % 3]), n, w, h, 0, 4095.0, _p(g0), _p(m0), 2.2, None, _p(targets[k % 3]),
+                                 _p(grads[k % 3]), _p(gp), hs())
+
+    def timed(fn):
+        for k in range(warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for k in range(steps):
+            fn(k)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    ms_f = timed(fwd)
+    ms_b = timed(bwd)
+    ms_fb = timed(lambda k: (fwd(k), bwd(k)))
+    px = n * w * h
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g, m, loss = C.fit_gains_ccm(raw, target, 0, 4095.0, 2.2, steps=200)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    err = max(float((g.cpu() - torch.tensor(g_t, dtype=torch.float32)).abs().max()),
+              float((m.cpu().reshape(9) - torch.tensor(m_t.ravel(), dtype=torch.float32)).abs().max()))
+    log(f"bench: chain fwd+loss {ms_f * 1e3:.1f} us, bwd {ms_b * 1e3:.1f} us, fwd+bwd {ms_fb * 1e3:.1f} us; "
+        f"200 Adam steps {fit_s:.2f} s, max param error {err:.2e}")
+    gbs = lambda nbytes, ms: round(nbytes / (ms / 1e3) / 1e9, 1)
+    out = {
+        "workload": "differentiable bilinear->WB->CCM->gamma chain, 8 x 1006x758 12-bit scene frames, FP32, MSE",
+        "pixels_per_step": px,
+        "value": round(px / (ms_fb / 1e3) / 1e6, 1),
+        "unit": "Mpix/s (forward+loss and backward per step)",
+        "ms_per_step": round(ms_fb, 5),
+        "forward_loss_us": round(ms_f * 1e3, 2),
+        "backward_us": round(ms_b * 1e3, 2),
+        "backward_over_forward": round(ms_b / ms_f, 3),
+        "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "forward_loss": {"algorithmic_bytes": 16 * px, "achieved": gbs(16 * px, ms_f),
+                                      "frac": round(gbs(16 * px, ms_f) / HBM_PEAK_GBS, 4)},
+                     "backward": {"algorithmic_bytes": 20 * px, "achieved": gbs(20 * px, ms_b),
+                                  "frac": round(gbs(20 * px, ms_b) / HBM_PEAK_GBS, 4)}},
+        "adam_200_steps": {"seconds": round(fit_s, 3), "max_param_error": err, "final_loss": float(loss)},
+        "steps": steps,
+        "warmup": warmup,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        raw1 = raw[:1].cpu().numpy()
+        t1 = time.perf_counter()
+        O.chain_fwd64(raw1, 0, 4095.0, g_t, m_t, 2.2, target[:1].cpu().numpy())
+        O.chain_vjp64(raw1, 0, 4095.0, g_t, m_t, 2.2, target=target[:1].cpu().numpy())
+        dt = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(w * h / dt / 1e6, 2), "unit": "Mpix/s (forward+loss and backward)",
+                               "cores": 1, "kind": "port",
+                               "sample": "one 1006x758 frame through the FP64 oracle's forward and VJP "
+                                         "(the reference has no backward), single thread"}
+    return out
+
+
 def cpu_baseline_batch(spec, raws, budget_s=10.0):
     """The reference CPU path over the batch's frames in order, for about
     budget_s seconds: pixels done / time."""
@@ -284,6 +455,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
+    ap.add_argument("--only", choices=("batch", "chain"), default=None,
+                    help="run only the batch or the chain measurement and print its JSON object (for profiling)")
+    ap.add_argument("--batch-shape", default=None,
+                    help="rows,waves_per_block,blocks_per_cu,order for the batched kernel instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed loop, run 4 plain device copies of this many MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -299,6 +475,18 @@ def main():
 
     import imop_amd
     from imop_amd import scene
+
+    if args.only == "batch":
+        shape = None if args.batch_shape is None else [int(v) for v in args.batch_shape.split(",")]
+        out = bench_batch(args.batch_steps, 3, False, shape)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "chain":
+        out = bench_chain(args.steps, args.warmup, False)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
 
     w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[args.workload]
     blv = None if bl is None else [bl] * 4
@@ -390,14 +578,7 @@ def main():
             "traffic_source": traffic_src,
         },
     }
-    if args.copy_mb > 0:
-        n = args.copy_mb << 20
-        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
-        dst = torch.empty(n, dtype=torch.uint8, device=dev)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"bench: {4} calibration copies of {n} bytes done")
+    calibration_copies(args.copy_mb)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(raw, args.workload)
@@ -405,6 +586,13 @@ def main():
         del d_in, d_out
         torch.cuda.empty_cache()
         result["batched"] = bench_batch(args.batch_steps, 3, not args.no_cpu_baseline)
+    if not args.no_batch and args.workload == "mg12":
+        torch.cuda.empty_cache()
+        result["prism"] = bench_single("prism10", 20, 3)
+        result["hqli_small"] = bench_single("hqli8", 200, 20)
+    if not args.no_chain and args.workload == "mg12":
+        torch.cuda.empty_cache()
+        result["chain"] = bench_chain(50, 5, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 
