@@ -25,23 +25,27 @@ namespace imop_hip {
 namespace {
 
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
-__device__ __forceinline__ void batch_item(const FastParams& f, int x, int y0, bool lane_out, bool careful) {
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out);
-    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out);
+__device__ __forceinline__ void batch_item(const FastParams& f, int x, int y0, bool lane_out, bool careful,
+                                           uint32_t* stage) {
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out, stage);
 }
 
 template <int FMT, int PAT, int ALG, int TH>
-__device__ __forceinline__ void batch_fmt(const FastParams& f, int x, int y0, bool lane_out, bool careful) {
+__device__ __forceinline__ void batch_fmt(const FastParams& f, int x, int y0, bool lane_out, bool careful,
+                                          uint32_t* stage) {
     // MG: 8-bit frames take the reference's integer path, deeper ones the exact rational one
     constexpr int MODE = ALG == 0 ? MODE_HQLI : (FMT == FMT_U8 ? MODE_MG_INT8 : MODE_MG_EXACT);
-    if (f.w & 2) batch_item<FMT, PAT, MODE, TH, true>(f, x, y0, lane_out, careful);
-    else batch_item<FMT, PAT, MODE, TH, false>(f, x, y0, lane_out, careful);
+    if (f.w & 2) batch_item<FMT, PAT, MODE, TH, true>(f, x, y0, lane_out, careful, stage);
+    else batch_item<FMT, PAT, MODE, TH, false>(f, x, y0, lane_out, careful, stage);
 }
 
 }  // namespace
 
 template <int PAT, int ALG, int TH>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
+    __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
+    uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // S slices, one per XCD (fewer when the grid has fewer than 8
@@ -94,10 +98,10 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         const int xs0 = sx * 248, xs1 = xs0 + 248;
         const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
         switch (bf.fmt) {
-        case FMT_U8: batch_fmt<FMT_U8, PAT, ALG, TH>(f, x, y0, lane_out, careful); break;
-        case FMT_U16: batch_fmt<FMT_U16, PAT, ALG, TH>(f, x, y0, lane_out, careful); break;
-        case FMT_P12: batch_fmt<FMT_P12, PAT, ALG, TH>(f, x, y0, lane_out, careful); break;
-        default: batch_fmt<FMT_P10, PAT, ALG, TH>(f, x, y0, lane_out, careful); break;
+        case FMT_U8: batch_fmt<FMT_U8, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
+        case FMT_U16: batch_fmt<FMT_U16, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
+        case FMT_P12: batch_fmt<FMT_P12, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
+        default: batch_fmt<FMT_P10, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
         }
     }
 }

@@ -11,8 +11,9 @@
 // a sliding window of 6 raw rows and 4 rows of MG colour differences in
 // registers, so every packed byte is loaded once per band (plus a 6-row
 // apron) with one aligned 4- or 8-byte load per lane and row, and every
-// output row is written once with 8-byte stores, coalesced across the wave.
-// No LDS and no barriers.
+// output row is written once: in interior bands regrouped through a 1.5 KB
+// LDS slice per wave into lane-contiguous 16-byte stores, in edge bands with
+// direct per-lane stores.  No barriers across waves.
 //
 // Arithmetic is the exact integer form of the reference algorithms (see
 // plain_kernels.hip): MG in units of 1/32 (float path) or the reference's own
@@ -250,6 +251,57 @@ __device__ __forceinline__ void store4(const FastParams& f, uint32_t x, uint32_t
     }
 }
 
+// Interior rows: the strip row's RGB bytes (62 producing lanes x 4 px) are
+// regrouped through a per-wave LDS slice so that the global stores are
+// 16-byte, lane-contiguous and write each cache line once (the direct
+// 24- / 12-byte-per-lane stores of store4 touch every line twice).  seg_px =
+// index of the strip's first pixel (y * w + strip start).  u16 output: 1488
+// bytes = 64 + 29 lanes x 16 B; u8 output (w % 4 == 0 only): 744 bytes =
+// 46 lanes x 16 B + 8 B.  LDS accesses of one wave execute in order; the
+// wave-scope fences only keep the compiler from reordering them.
+__device__ __forceinline__ void lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* stage, int lane, uint32_t seg_px,
+                                                 const int r[4], const int g[4], const int b[4]) {
+    const bool prod = lane >= 1 && lane <= 62;
+    if (f.out16) {
+        if (prod) {
+            uint2* p = (uint2*)(stage + 6 * (lane - 1));
+            p[0] = make_uint2((uint32_t)r[0] | ((uint32_t)g[0] << 16), (uint32_t)b[0] | ((uint32_t)r[1] << 16));
+            p[1] = make_uint2((uint32_t)g[1] | ((uint32_t)b[1] << 16), (uint32_t)r[2] | ((uint32_t)g[2] << 16));
+            p[2] = make_uint2((uint32_t)b[2] | ((uint32_t)r[3] << 16), (uint32_t)g[3] | ((uint32_t)b[3] << 16));
+        }
+        lds_order();
+        const uint4 v0 = ((const uint4*)stage)[lane];
+        uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
+        if (lane < 29) v1 = ((const uint4*)stage)[64 + lane];
+        lds_order();
+        uint4* o = (uint4*)(f.out + 6u * seg_px);
+        o[lane] = v0;
+        if (lane < 29) o[64 + lane] = v1;
+    } else {
+        if (prod) {
+            uint32_t* p = stage + 3 * (lane - 1);
+            p[0] = (uint32_t)r[0] | ((uint32_t)g[0] << 8) | ((uint32_t)b[0] << 16) | ((uint32_t)r[1] << 24);
+            p[1] = (uint32_t)g[1] | ((uint32_t)b[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)g[2] << 24);
+            p[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
+        }
+        lds_order();
+        uint4 v0 = make_uint4(0u, 0u, 0u, 0u);
+        uint2 v1 = make_uint2(0u, 0u);
+        if (lane < 46) v0 = ((const uint4*)stage)[lane];
+        if (lane == 46) v1 = ((const uint2*)stage)[92];
+        lds_order();
+        uint8_t* base = f.out + 3u * seg_px;
+        if (lane < 46) ((uint4*)base)[lane] = v0;
+        if (lane == 46) ((uint2*)base)[92] = v1;
+    }
+}
+
 // MG colour difference of the two R/B sites a lane owns in one row.
 // P points at 5 window rows centred on the row (P[0] two rows up).
 // Halo columns come from the neighbouring lanes.
@@ -335,7 +387,7 @@ __device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const 
 
 // One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
 template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD>
-__device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, bool lane_out) {
+__device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, bool lane_out, uint32_t* stage) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
     const int w = f.w, h = f.h;
     const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
@@ -461,7 +513,14 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
         for (int a = 0; a < 2; ++a) {
             const int y = yA + a;
             if (!CAREFUL) {
-                if (lane_out) store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
+                // staged rows need 4-byte aligned runs: always for u16, for u8 when w % 4 == 0
+                if (!ODD || f.out16) {
+                    const int lane = threadIdx.x & 63;
+                    store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)(x + 4 - 4 * lane), R[a],
+                                     G[a], B[a]);
+                } else if (lane_out) {
+                    store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
+                }
                 continue;
             }
             if (y >= h || y < 2 || y >= h - 2 || !lane_out) continue;
@@ -492,6 +551,7 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
 // shared by vertically adjacent bands are read through one L2.
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
 __global__ __launch_bounds__(256) void k_fast(FastParams f) {
+    __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     const int nblk = gridDim.x;
@@ -515,8 +575,9 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     // careful: the band's outputs touch the 3-pixel frame rim, or its raw
     // window reaches the stream's final row
     const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out);
-    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out);
+    uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out, stage);
 }
 
 // ---------------------------------------------------------------- launchers
