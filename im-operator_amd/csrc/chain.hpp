@@ -30,5 +30,7 @@ hipError_t launch_chain_reduce(const float* partials, int G, int m, float scale,
 hipError_t launch_bilinear_fwd(const float* x, int w, int h, int pattern, float* out, hipStream_t st);
 hipError_t launch_bilinear_vjp(const float* v, int w, int h, int pattern, float* out, hipStream_t st);
 int chain_bwd_tiles(int n, int w, int h);
+// Workgroups (= partial sums) of the forward for a partial-sum budget.
+int chain_fwd_blocks(int n, int w, int h, int budget);
 
 }  // namespace imop_hip

@@ -34,8 +34,9 @@ namespace imop_hip {
 
 namespace {
 
-constexpr int kTW = 64;   // backward tile: 64 x 8 raw pixels per iteration
-constexpr int kTH = 8;
+constexpr int kTW = 64;   // backward tile: 64 x 16 raw pixels per iteration
+constexpr int kTH = 16;
+constexpr int kFW = 256;  // forward: row segments of 256 pixels per workgroup
 
 __device__ __forceinline__ bool interior(int x, int y, int w, int h) {
     return x >= 2 && x < w - 2 && y >= 2 && y < h - 2;
@@ -80,7 +81,11 @@ struct Px {
     float a[3], b[3], u[3], c[3], z[3], y[3];
 };
 
-__device__ __forceinline__ float gamma_pow(float z, float e) { return z > 0.0f ? exp2f(e * log2f(z)) : 0.0f; }
+// z^e for z in [0, 1] with the hardware log2 / exp2 (v_log_f32, v_exp_f32,
+// about 1 ulp each); 0 at z = 0 (and for denormal z, which v_log flushes).
+__device__ __forceinline__ float gamma_pow(float z, float e) {
+    return z > 0.0f ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(z)) : 0.0f;
+}
 
 __device__ __forceinline__ void chain_px(const float a[3], const float g[3], const float M[9], float e, Px& s) {
 #pragma unroll
@@ -104,7 +109,8 @@ __device__ __forceinline__ void chain_px_bwd(const Px& s, const float gy[3], con
     float gc[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float dz = s.z[i] > 0.0f ? e * s.y[i] / s.z[i] : 0.0f;
+        // d/dz z^e = e z^(e-1) = e y / z  (0 where y is 0: z = 0 or flushed)
+        const float dz = s.y[i] > 0.0f ? e * s.y[i] * __builtin_amdgcn_rcpf(s.z[i]) : 0.0f;
         const bool pass = s.c[i] >= 0.0f && s.c[i] <= 1.0f;
         gc[i] = pass ? gy[i] * dz : 0.0f;
     }
@@ -164,20 +170,21 @@ __device__ __forceinline__ Params load_params(const float* gains, const float* c
 // ------------------------------------------------------------------ forward
 
 __global__ __launch_bounds__(256) void k_chain_fwd(ChainArgs A) {
+    // workgroup (bx, by): columns bx * 256 + tid of rows by, by + gridDim.y, ...
+    // over the n * h rows of the batch
     const Params P = load_params(A.gains, A.ccm);
-    const int64_t wh = (int64_t)A.w * A.h, total = wh * A.n;
+    const int w = A.w, h = A.h;
+    const int x = blockIdx.x * kFW + threadIdx.x;
+    const float im = A.inv_max;
     float acc[1] = {0.0f};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int f = (int)(i / wh);
-        const int rem = (int)(i - (int64_t)f * wh);
-        const int y = rem / A.w, x = rem - y * A.w;
-        const float* fr = A.raw + (int64_t)f * wh;
+    for (int ry = blockIdx.y; ry < A.n * h; ry += gridDim.y) {
+        if (x >= w) continue;
+        const int f = ry / h, y = ry - f * h;
+        const int64_t i = (int64_t)ry * w + x;  // pixel index in the batch
         float a[3] = {0.0f, 0.0f, 0.0f};
         int sx, sy;
-        if (src_of(x, y, A.w, A.h, sx, sy)) {
-            const float* c = fr + (int64_t)sy * A.w + sx;
-            const int w = A.w;
-            const float im = A.inv_max;
+        if (src_of(x, y, w, h, sx, sy)) {
+            const float* c = A.raw + ((int64_t)f * h + sy) * w + sx;
             bilinear_at(A.pattern, sx, sy, [&](int dx, int dy) { return c[dy * w + dx] * im; }, a);
         }
         Px s;
@@ -194,7 +201,7 @@ __global__ __launch_bounds__(256) void k_chain_fwd(ChainArgs A) {
             }
         }
     }
-    if (A.target) block_sum<1>(acc, A.partials + blockIdx.x);
+    if (A.target) block_sum<1>(acc, A.partials + (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // ----------------------------------------------------------------- backward
@@ -351,8 +358,17 @@ __global__ __launch_bounds__(256) void k_bilinear_vjp(const float* v, int w, int
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_chain_fwd(const ChainArgs& a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_chain_fwd, dim3(blocks), dim3(256), 0, st, a);
+    // blocks = the partial-sum budget; spread over row segments x row groups
+    const int gx = (a.w + kFW - 1) / kFW;
+    const int rows = a.n * a.h;
+    const int gy = std::max(1, std::min(rows, blocks / gx));
+    hipLaunchKernelGGL(k_chain_fwd, dim3(gx, gy), dim3(kFW), 0, st, a);
     return hipGetLastError();
+}
+
+int chain_fwd_blocks(int n, int w, int h, int budget) {
+    const int gx = (w + kFW - 1) / kFW;
+    return gx * std::max(1, std::min(n * h, budget / gx));
 }
 
 hipError_t launch_chain_bwd(const ChainArgs& a, int blocks, hipStream_t st) {

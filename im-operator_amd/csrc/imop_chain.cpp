@@ -89,9 +89,10 @@ int imop_hip_isp_chain_fwd(const float* d_raw, int n, int width, int height, int
     if (partials_for_device(&partials) != hipSuccess) return kInternal;
     a.partials = partials;
     const int64_t total = (int64_t)n * width * height;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, kMaxBlocks);
+    if ((width + 255) / 256 > kMaxBlocks) return kDims;
+    const int blocks = chain_fwd_blocks(n, width, height, kMaxBlocks);
     hipStream_t st = (hipStream_t)stream;
-    if (launch_chain_fwd(a, blocks, st) != hipSuccess) return kInternal;
+    if (launch_chain_fwd(a, kMaxBlocks, st) != hipSuccess) return kInternal;
     if (d_target) {
         const float scale = (float)(1.0 / ((double)total * 3.0));
         if (launch_chain_reduce(partials, blocks, 1, scale, d_loss, st) != hipSuccess) return kInternal;
