@@ -170,14 +170,19 @@ __device__ __forceinline__ Params load_params(const float* gains, const float* c
 // ------------------------------------------------------------------ forward
 
 __global__ __launch_bounds__(256) void k_chain_fwd(ChainArgs A) {
-    // workgroup (bx, by): columns bx * 256 + tid of rows by, by + gridDim.y, ...
-    // over the n * h rows of the batch
+    // workgroup (bx, by): columns bx * 256 + tid of rows [by * per, (by + 1) * per)
+    // of the n * h rows of the batch
     const Params P = load_params(A.gains, A.ccm);
     const int w = A.w, h = A.h;
     const int x = blockIdx.x * kFW + threadIdx.x;
     const float im = A.inv_max;
     float acc[1] = {0.0f};
-    for (int ry = blockIdx.y; ry < A.n * h; ry += gridDim.y) {
+    // a contiguous run of rows per workgroup, so the 3-row neighbourhoods are
+    // re-read from the caches rather than from HBM
+    const int rows = A.n * h;
+    const int per = (rows + gridDim.y - 1) / gridDim.y;
+    const int r1 = min(rows, (int)(blockIdx.y + 1) * per);
+    for (int ry = blockIdx.y * per; ry < r1; ++ry) {
         if (x >= w) continue;
         const int f = ry / h, y = ry - f * h;
         const int64_t i = (int64_t)ry * w + x;  // pixel index in the batch
@@ -299,14 +304,22 @@ __global__ __launch_bounds__(256) void k_chain_bwd(ChainArgs A) {
     block_sum<12>(acc, A.partials + (int64_t)blockIdx.x * 12);
 }
 
-// Sum G partial rows of m values in a fixed order: one wave per value.
-__global__ __launch_bounds__(768) void k_chain_reduce(const float* partials, int G, int m, float scale, float* out) {
-    const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (v >= m) return;
-    float s = 0.0f;
-    for (int k = lane; k < G; k += 64) s += partials[(int64_t)k * m + v];
-    s = wave_sum(s);
-    if (lane == 0) out[v] = s * scale;
+// Sum G (<= 2048) partial rows of m values in a fixed order: workgroup v
+// sums value v, each thread 8 partials loaded together.
+__global__ __launch_bounds__(256) void k_chain_reduce(const float* __restrict__ partials, int G, int m, float scale,
+                                                      float* __restrict__ out) {
+    const int v = blockIdx.x;
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int k = threadIdx.x + u * 256;
+        x[u] = k < G ? partials[(int64_t)k * m + v] : 0.0f;
+    }
+    __shared__ float red[4];
+    const float s = wave_sum(((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7])));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[v] = ((red[0] + red[1]) + (red[2] + red[3])) * scale;
 }
 
 // ---------------------------------------------------- the demosaic operator
@@ -377,7 +390,7 @@ hipError_t launch_chain_bwd(const ChainArgs& a, int blocks, hipStream_t st) {
 }
 
 hipError_t launch_chain_reduce(const float* partials, int G, int m, float scale, float* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_chain_reduce, dim3(1), dim3(64 * m), 0, st, partials, G, m, scale, out);
+    hipLaunchKernelGGL(k_chain_reduce, dim3(m), dim3(256), 0, st, partials, G, m, scale, out);
     return hipGetLastError();
 }
 
