@@ -1,13 +1,30 @@
 // prism_kernels.hip -- PRISM edge-directed demosaic (src/algorithms/prism.cpp:
-// 25-316) on gfx950: decode + black level, then the reference's stages as
-// six stencil passes over FP32 planes, then the border fill and rounding
-// fused into the interleaved-RGB write.
+// 25-316) on gfx950 in two LDS-tiled passes.
 //
-// Float reproducibility: every stage keeps the reference's operation order.
-// The reference library is compiled by GCC with -mavx2 -mfma, which fuses the
-// first product of each a*b + c*d sum into one multiply-add; those sites use
-// fmaf() here and every other operation is a separately rounded IEEE op
-// (contraction is off for this file; division is correctly rounded).
+//   k_prism_a  raw -> CFA (decode + black level, S0), VH map (S1), low-pass
+//              (S2), PQ map (S4) and green at R/B (S3); writes four FP32
+//              planes: cfa, vh, pq, c1.
+//   k_prism_b  those planes -> R/B at B/R (S5), R and B at G (S6), the 3x3
+//              colour-difference medians (S7-S9), the width-4 border fill,
+//              rounding and the interleaved RGB write (S10, S11).
+//
+// Each workgroup owns a 64 x 16 tile of output pixels and computes every
+// intermediate it needs over the tile plus an apron in LDS: S3 reads CFA up
+// to 5 pixels away (VH at +-1 of a +-4 window), and S5 -> S6 -> median reach
+// 7 pixels into the four planes.  The last tile of each row / column is
+// shifted inward so that every tile lies inside the frame; the width-4 border
+// fill then always finds its source pixel (at most 4 pixels inward) in the
+// same tile, and overlapping tiles write identical values.
+//
+// Float reproducibility: every stage keeps the reference's operation order
+// (the stage functions take accessors, so the same expressions serve LDS and
+// global planes).  The reference library is compiled by GCC with -mavx2
+// -mfma, which fuses the first product of each a*b + c*d sum into one
+// multiply-add; those sites use fmaf() here and every other operation is a
+// separately rounded IEEE op (contraction is off for this file; division is
+// correctly rounded).  Regions where the reference leaves a plane at its
+// initial value (VH = PQ = 0.5, low-pass 0, chroma natives / 0 outside
+// [4, h-4) x [4, w-4)) keep those values.
 #include "device_common.hpp"
 #include "plain.hpp"
 
@@ -20,8 +37,10 @@ namespace {
 constexpr float kEps = 1e-5f;
 constexpr float kEpsSq = 1e-10f;
 
+constexpr int TW = 64, TH = 16;  // output tile
+
 struct PrismPlanes {
-    float *cfa, *c0, *c1, *c2, *vh, *pq, *lpf, *r9, *b9;
+    float *cfa, *c1, *vh, *pq;
 };
 
 __host__ __device__ inline size_t plane_stride(int w, int h) {
@@ -32,7 +51,7 @@ __host__ __device__ inline size_t plane_stride(int w, int h) {
 __device__ __forceinline__ PrismPlanes planes_of(void* ws, int w, int h) {
     const size_t s = plane_stride(w, h);
     float* b = (float*)ws;
-    return PrismPlanes{b, b + s, b + 2 * s, b + 3 * s, b + 4 * s, b + 5 * s, b + 6 * s, b + 7 * s, b + 8 * s};
+    return PrismPlanes{b, b + s, b + 2 * s, b + 3 * s};
 }
 
 // std::min / std::max argument order: first argument wins ties.
@@ -40,168 +59,117 @@ __device__ __forceinline__ float fmin_first(float a, float b) { return (b < a) ?
 __device__ __forceinline__ float fmax_first(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float clamp01(float v) { return fmax_first(0.0f, fmin_first(1.0f, v)); }
 
-__device__ __forceinline__ bool xy_of(int& x, int& y, int w, int h) {
-    x = blockIdx.x * blockDim.x + threadIdx.x;
-    y = blockIdx.y * blockDim.y + threadIdx.y;
-    return x < w && y < h;
+__device__ __forceinline__ bool inside(int x, int y, int w, int h, int b) {
+    return y >= b && y < h - b && x >= b && x < w - b;
 }
 
-// S0 + S1 + S2 + S4: normalised CFA, native planes, VH / PQ direction maps and
-// the R/B low-pass.  S1/S2/S4 read only the CFA, which S0 computes here from
-// the raw samples, so one pass covers them.
-__global__ __launch_bounds__(256) void k_prism_s0(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
-    const int w = f.w, h = f.h;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    const float scale = (float)f.maxv;
-    const int s = site_of(f.pattern, y, x);
-    const float v = (float)sample_at(f, x, y) / scale;
-    P.cfa[i] = v;
-    P.c0[i] = s == 0 ? v : 0.0f;
-    P.c1[i] = s == 1 ? v : 0.0f;
-    P.c2[i] = s == 2 ? v : 0.0f;
+// Tile origin; the last tile of a row / column is shifted inward.
+__device__ __forceinline__ void tile_origin(int w, int h, int& x0, int& y0) {
+    x0 = imin((int)blockIdx.x * TW, imax(w - TW, 0));
+    y0 = imin((int)blockIdx.y * TH, imax(h - TH, 0));
 }
 
-__global__ __launch_bounds__(256) void k_prism_dirs(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
-    const int w = f.w, h = f.h;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    const float* cfa = P.cfa;
-    const int s = site_of(f.pattern, y, x);
-    // S1 (prism.cpp:84-99)
-    float vh = 0.5f;
-    if (y >= 4 && y < h - 4 && x >= 4 && x < w - 4) {
-        float vs = 0.0f, hs = 0.0f;
-        for (int k = -4; k <= 4; ++k) {
-            vs += fabsf(cfa[i] - cfa[i + (ptrdiff_t)k * w]);
-            hs += fabsf(cfa[i] - cfa[i + k]);
-        }
-        vs = fmax_first(vs, kEpsSq);
-        hs = fmax_first(hs, kEpsSq);
-        vh = vs / (vs + hs);
+// ------------------------------------------------------------ stage bodies
+// C(dx, dy): CFA at the pixel's offset; likewise the other accessors.
+
+// S1 (prism.cpp:84-99), inside [4, h-4) x [4, w-4)
+template <class Cf>
+__device__ __forceinline__ float stage_vh(Cf C) {
+    const float c = C(0, 0);
+    float vs = 0.0f, hs = 0.0f;
+    for (int k = -4; k <= 4; ++k) {
+        vs += fabsf(c - C(0, k));
+        hs += fabsf(c - C(k, 0));
     }
-    P.vh[i] = vh;
-    // S2 (prism.cpp:101-112)
-    float lp = 0.0f;
-    if (s != 1 && y >= 2 && y < h - 2 && x >= 2 && x < w - 2) {
-        float s4 = cfa[i - w] + cfa[i + w] + cfa[i - 1] + cfa[i + 1];
-        float d4 = cfa[i - w - 1] + cfa[i - w + 1] + cfa[i + w - 1] + cfa[i + w + 1];
-        lp = fmaf(0.0625f, d4, fmaf(0.25f, cfa[i], 0.125f * s4));
-    }
-    P.lpf[i] = lp;
-    // S4 (prism.cpp:165-179)
-    float pq = 0.5f;
-    if (s != 1 && y >= 4 && y < h - 4 && x >= 4 && x < w - 4) {
-        float ps = 0.0f, qs = 0.0f;
-        for (int k = -4; k <= 4; ++k) {
-            ps += fabsf(cfa[i] - cfa[i + (ptrdiff_t)k * w + k]);
-            qs += fabsf(cfa[i] - cfa[i + (ptrdiff_t)k * w - k]);
-        }
-        ps = fmax_first(ps, kEpsSq);
-        qs = fmax_first(qs, kEpsSq);
-        pq = ps / (ps + qs);
-    }
-    P.pq[i] = pq;
+    vs = fmax_first(vs, kEpsSq);
+    hs = fmax_first(hs, kEpsSq);
+    return vs / (vs + hs);
 }
 
-// S3: green at R/B sites (prism.cpp:114-163)
-__global__ __launch_bounds__(256) void k_prism_s3(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
-    const int w = f.w, h = f.h;
-    if (!(y >= 4 && y < h - 4 && x >= 4 && x < w - 4) || site_of(f.pattern, y, x) == 1) return;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    const ptrdiff_t W = w;
-    const float* cfa = P.cfa;
-    const float* VH = P.vh;
-    const float* lpf = P.lpf;
-    float vc = VH[i];
-    float vn = 0.25f * (VH[i - W - 1] + VH[i - W + 1] + VH[i + W - 1] + VH[i + W + 1]);
+// S2 (prism.cpp:101-112), R/B sites inside [2, h-2) x [2, w-2)
+template <class Cf>
+__device__ __forceinline__ float stage_lpf(Cf C) {
+    float s4 = C(0, -1) + C(0, 1) + C(-1, 0) + C(1, 0);
+    float d4 = C(-1, -1) + C(1, -1) + C(-1, 1) + C(1, 1);
+    return fmaf(0.0625f, d4, fmaf(0.25f, C(0, 0), 0.125f * s4));
+}
+
+// S4 (prism.cpp:165-179), R/B sites inside [4, h-4) x [4, w-4)
+template <class Cf>
+__device__ __forceinline__ float stage_pq(Cf C) {
+    const float c = C(0, 0);
+    float ps = 0.0f, qs = 0.0f;
+    for (int k = -4; k <= 4; ++k) {
+        ps += fabsf(c - C(k, k));
+        qs += fabsf(c - C(-k, k));
+    }
+    ps = fmax_first(ps, kEpsSq);
+    qs = fmax_first(qs, kEpsSq);
+    return ps / (ps + qs);
+}
+
+// S3: green at R/B sites (prism.cpp:114-163), inside [4, h-4) x [4, w-4)
+template <class Cf, class Vf, class Lf>
+__device__ __forceinline__ float stage_s3(Cf C, Vf V, Lf L) {
+    float vc = V(0, 0);
+    float vn = 0.25f * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1));
     float disc = (fabsf(0.5f - vc) < fabsf(0.5f - vn)) ? vn : vc;
-    float c0 = cfa[i];
-    float gN = kEps + fabsf(cfa[i - W] - cfa[i + W]) + fabsf(c0 - cfa[i - 2 * W]) + fabsf(cfa[i - W] - cfa[i - 3 * W]);
-    float gS = kEps + fabsf(cfa[i + W] - cfa[i - W]) + fabsf(c0 - cfa[i + 2 * W]) + fabsf(cfa[i + W] - cfa[i + 3 * W]);
-    float gW = kEps + fabsf(cfa[i - 1] - cfa[i + 1]) + fabsf(c0 - cfa[i - 2]) + fabsf(cfa[i - 1] - cfa[i - 3]);
-    float gE = kEps + fabsf(cfa[i + 1] - cfa[i - 1]) + fabsf(c0 - cfa[i + 2]) + fabsf(cfa[i + 1] - cfa[i + 3]);
-    float l0 = lpf[i], lN = lpf[i - 2 * W], lS = lpf[i + 2 * W], lW = lpf[i - 2], lE = lpf[i + 2];
-    float eN = cfa[i - W] * (1.0f + (l0 - lN) / (kEps + l0 + lN));
-    float eS = cfa[i + W] * (1.0f + (l0 - lS) / (kEps + l0 + lS));
-    float eW = cfa[i - 1] * (1.0f + (l0 - lW) / (kEps + l0 + lW));
-    float eE = cfa[i + 1] * (1.0f + (l0 - lE) / (kEps + l0 + lE));
+    float c0 = C(0, 0);
+    float gN = kEps + fabsf(C(0, -1) - C(0, 1)) + fabsf(c0 - C(0, -2)) + fabsf(C(0, -1) - C(0, -3));
+    float gS = kEps + fabsf(C(0, 1) - C(0, -1)) + fabsf(c0 - C(0, 2)) + fabsf(C(0, 1) - C(0, 3));
+    float gW = kEps + fabsf(C(-1, 0) - C(1, 0)) + fabsf(c0 - C(-2, 0)) + fabsf(C(-1, 0) - C(-3, 0));
+    float gE = kEps + fabsf(C(1, 0) - C(-1, 0)) + fabsf(c0 - C(2, 0)) + fabsf(C(1, 0) - C(3, 0));
+    float l0 = L(0, 0), lN = L(0, -2), lS = L(0, 2), lW = L(-2, 0), lE = L(2, 0);
+    float eN = C(0, -1) * (1.0f + (l0 - lN) / (kEps + l0 + lN));
+    float eS = C(0, 1) * (1.0f + (l0 - lS) / (kEps + l0 + lS));
+    float eW = C(-1, 0) * (1.0f + (l0 - lW) / (kEps + l0 + lW));
+    float eE = C(1, 0) * (1.0f + (l0 - lE) / (kEps + l0 + lE));
     float vEst = fmaf(gS, eN, gN * eS) / (gN + gS);
     float hEst = fmaf(gW, eE, gE * eW) / (gE + gW);
-    float lapV = (2.0f * c0 - cfa[i - 2 * W] - cfa[i + 2 * W]) * 0.25f;
-    float lapH = (2.0f * c0 - cfa[i - 2] - cfa[i + 2]) * 0.25f;
+    float lapV = (2.0f * c0 - C(0, -2) - C(0, 2)) * 0.25f;
+    float lapH = (2.0f * c0 - C(-2, 0) - C(2, 0)) * 0.25f;
     vEst += lapV;
     hEst += lapH;
-    P.c1[i] = clamp01(fmaf(disc, hEst, (1.0f - disc) * vEst));
+    return clamp01(fmaf(disc, hEst, (1.0f - disc) * vEst));
 }
 
-// S5: opposite chroma at R/B sites (prism.cpp:181-225)
-__global__ __launch_bounds__(256) void k_prism_s5(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
-    const int w = f.w, h = f.h;
-    const int s = site_of(f.pattern, y, x);
-    if (!(y >= 4 && y < h - 4 && x >= 4 && x < w - 4) || s == 1) return;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    const ptrdiff_t W = w;
-    float* X = (s == 0) ? P.c2 : P.c0;
-    const float* G = P.c1;
-    const float* PQ = P.pq;
-    float pc = PQ[i];
-    float pn = 0.25f * (PQ[i - W - 1] + PQ[i - W + 1] + PQ[i + W - 1] + PQ[i + W + 1]);
+// S5: the opposite chroma at an R/B site (prism.cpp:181-225), inside
+// [4, h-4) x [4, w-4).  X = the opposite colour's plane, G = green.
+template <class Xf, class Gf, class Qf>
+__device__ __forceinline__ float stage_s5(Xf X, Gf G, Qf Q) {
+    float pc = Q(0, 0);
+    float pn = 0.25f * (Q(-1, -1) + Q(1, -1) + Q(-1, 1) + Q(1, 1));
     float disc = (fabsf(0.5f - pc) < fabsf(0.5f - pn)) ? pn : pc;
-    float g0 = G[i];
-    float gNW = kEps + fabsf(X[i - W - 1] - X[i + W + 1]) + fabsf(g0 - G[i - 2 * W - 2]) + fabsf(X[i - W - 1] - X[i - 3 * W - 3]);
-    float gNE = kEps + fabsf(X[i - W + 1] - X[i + W - 1]) + fabsf(g0 - G[i - 2 * W + 2]) + fabsf(X[i - W + 1] - X[i - 3 * W + 3]);
-    float gSW = kEps + fabsf(X[i + W - 1] - X[i - W + 1]) + fabsf(g0 - G[i + 2 * W - 2]) + fabsf(X[i + W - 1] - X[i + 3 * W - 3]);
-    float gSE = kEps + fabsf(X[i + W + 1] - X[i - W - 1]) + fabsf(g0 - G[i + 2 * W + 2]) + fabsf(X[i + W + 1] - X[i + 3 * W + 3]);
-    float eNW = X[i - W - 1] - G[i - W - 1];
-    float eNE = X[i - W + 1] - G[i - W + 1];
-    float eSW = X[i + W - 1] - G[i + W - 1];
-    float eSE = X[i + W + 1] - G[i + W + 1];
+    float g0 = G(0, 0);
+    float gNW = kEps + fabsf(X(-1, -1) - X(1, 1)) + fabsf(g0 - G(-2, -2)) + fabsf(X(-1, -1) - X(-3, -3));
+    float gNE = kEps + fabsf(X(1, -1) - X(-1, 1)) + fabsf(g0 - G(2, -2)) + fabsf(X(1, -1) - X(3, -3));
+    float gSW = kEps + fabsf(X(-1, 1) - X(1, -1)) + fabsf(g0 - G(-2, 2)) + fabsf(X(-1, 1) - X(-3, 3));
+    float gSE = kEps + fabsf(X(1, 1) - X(-1, -1)) + fabsf(g0 - G(2, 2)) + fabsf(X(1, 1) - X(3, 3));
+    float eNW = X(-1, -1) - G(-1, -1);
+    float eNE = X(1, -1) - G(1, -1);
+    float eSW = X(-1, 1) - G(-1, 1);
+    float eSE = X(1, 1) - G(1, 1);
     float pEst = fmaf(gNW, eSE, gSE * eNW) / (gNW + gSE);
     float qEst = fmaf(gNE, eSW, gSW * eNE) / (gNE + gSW);
-    X[i] = clamp01(fmaf(disc, qEst, fmaf(1.0f - disc, pEst, g0)));
+    return clamp01(fmaf(disc, qEst, fmaf(1.0f - disc, pEst, g0)));
 }
 
-// S6: R and B at G sites (prism.cpp:227-264)
-__global__ __launch_bounds__(256) void k_prism_s6(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
-    const int w = f.w, h = f.h;
-    if (!(y >= 4 && y < h - 4 && x >= 4 && x < w - 4) || site_of(f.pattern, y, x) != 1) return;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    const ptrdiff_t W = w;
-    const float* G = P.c1;
-    const float* VH = P.vh;
-    float vc = VH[i];
-    float vn = 0.25f * (VH[i - W - 1] + VH[i - W + 1] + VH[i + W - 1] + VH[i + W + 1]);
-    float disc = (fabsf(0.5f - vc) < fabsf(0.5f - vn)) ? vn : vc;
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        float* X = ch == 0 ? P.c0 : P.c2;
-        float g0 = G[i];
-        float gN = kEps + fabsf(g0 - G[i - 2 * W]) + fabsf(X[i - W] - X[i + W]) + fabsf(X[i - W] - X[i - 3 * W]);
-        float gS = kEps + fabsf(g0 - G[i + 2 * W]) + fabsf(X[i + W] - X[i - W]) + fabsf(X[i + W] - X[i + 3 * W]);
-        float gW = kEps + fabsf(g0 - G[i - 2]) + fabsf(X[i - 1] - X[i + 1]) + fabsf(X[i - 1] - X[i - 3]);
-        float gE = kEps + fabsf(g0 - G[i + 2]) + fabsf(X[i + 1] - X[i - 1]) + fabsf(X[i + 1] - X[i + 3]);
-        float dN = X[i - W] - G[i - W];
-        float dS = X[i + W] - G[i + W];
-        float dW = X[i - 1] - G[i - 1];
-        float dE = X[i + 1] - G[i + 1];
-        float vEst = fmaf(gS, dN, gN * dS) / (gN + gS);
-        float hEst = fmaf(gW, dE, gE * dW) / (gE + gW);
-        X[i] = clamp01(fmaf(1.0f - disc, vEst, fmaf(disc, hEst, g0)));
-    }
+// S6: one chroma (X) at a G site (prism.cpp:227-264), inside
+// [4, h-4) x [4, w-4); disc from the VH map.
+template <class Xf, class Gf>
+__device__ __forceinline__ float stage_s6(Xf X, Gf G, float disc) {
+    float g0 = G(0, 0);
+    float gN = kEps + fabsf(g0 - G(0, -2)) + fabsf(X(0, -1) - X(0, 1)) + fabsf(X(0, -1) - X(0, -3));
+    float gS = kEps + fabsf(g0 - G(0, 2)) + fabsf(X(0, 1) - X(0, -1)) + fabsf(X(0, 1) - X(0, 3));
+    float gW = kEps + fabsf(g0 - G(-2, 0)) + fabsf(X(-1, 0) - X(1, 0)) + fabsf(X(-1, 0) - X(-3, 0));
+    float gE = kEps + fabsf(g0 - G(2, 0)) + fabsf(X(1, 0) - X(-1, 0)) + fabsf(X(1, 0) - X(3, 0));
+    float dN = X(0, -1) - G(0, -1);
+    float dS = X(0, 1) - G(0, 1);
+    float dW = X(-1, 0) - G(-1, 0);
+    float dE = X(1, 0) - G(1, 0);
+    float vEst = fmaf(gS, dN, gN * dS) / (gN + gS);
+    float hEst = fmaf(gW, dE, gE * dW) / (gE + gW);
+    return clamp01(fmaf(1.0f - disc, vEst, fmaf(disc, hEst, g0)));
 }
 
 // prism_detail::median_3x3 (prism.cpp:9-21): the same exchange network.
@@ -221,71 +189,201 @@ __device__ __forceinline__ float median9(float v[9]) {
     return v[4];
 }
 
-// S7 + S8 + S9: colour differences, their 3x3 medians, reconstruction
-// (prism.cpp:266-300).  The outer ring keeps the S6 chroma.
-__global__ __launch_bounds__(256) void k_prism_median(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
+// LDS plane of a tile with apron A: element (x, y) of the frame.
+template <int A>
+struct Tile {
+    static constexpr int W = TW + 2 * A, H = TH + 2 * A;
+};
+
+// ------------------------------------------------------------------ pass A
+
+constexpr int AC = 5;  // CFA apron of pass A
+
+__global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
+    __shared__ float s_cfa[Tile<AC>::H][Tile<AC>::W];  // tile + 5
+    __shared__ float s_vh[Tile<1>::H][Tile<1>::W];     // tile + 1
+    __shared__ float s_lp[Tile<2>::H][Tile<2>::W];     // tile + 2
     const int w = f.w, h = f.h;
-    PrismPlanes P = planes_of(ws, w, h);
-    const size_t i = (size_t)y * w + x;
-    if (!(y >= 1 && y < h - 1 && x >= 1 && x < w - 1)) {
-        P.r9[i] = P.c0[i];
-        P.b9[i] = P.c2[i];
-        return;
+    int x0, y0;
+    tile_origin(w, h, x0, y0);
+    const PrismPlanes P = planes_of(ws, w, h);
+    const float scale = (float)f.maxv;
+
+    // S0: normalised CFA (prism.cpp:58-78)
+    for (int k = threadIdx.x; k < Tile<AC>::H * Tile<AC>::W; k += blockDim.x) {
+        const int j = k / Tile<AC>::W, i = k - j * Tile<AC>::W;
+        const int x = x0 - AC + i, y = y0 - AC + j;
+        s_cfa[j][i] = (x >= 0 && x < w && y >= 0 && y < h) ? (float)sample_at(f, x, y) / scale : 0.0f;
     }
-    float a[9], b[9];
-    int k = 0;
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            size_t j = (size_t)(y + dy) * w + (size_t)(x + dx);
-            float g = P.c1[j];
-            a[k] = P.c0[j] - g;
-            b[k] = P.c2[j] - g;
-            ++k;
-        }
-    float lm = median9(a), mm = median9(b);
-    float g0 = P.c1[i];
-    P.r9[i] = clamp01(g0 + lm);
-    P.b9[i] = clamp01(g0 + mm);
+    __syncthreads();
+    // S1 over tile + 1, S2 over tile + 2
+    for (int k = threadIdx.x; k < Tile<1>::H * Tile<1>::W; k += blockDim.x) {
+        const int j = k / Tile<1>::W, i = k - j * Tile<1>::W;
+        const int x = x0 - 1 + i, y = y0 - 1 + j;
+        const int cj = j + AC - 1, ci = i + AC - 1;
+        s_vh[j][i] = inside(x, y, w, h, 4) ? stage_vh([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; }) : 0.5f;
+    }
+    for (int k = threadIdx.x; k < Tile<2>::H * Tile<2>::W; k += blockDim.x) {
+        const int j = k / Tile<2>::W, i = k - j * Tile<2>::W;
+        const int x = x0 - 2 + i, y = y0 - 2 + j;
+        const int cj = j + AC - 2, ci = i + AC - 2;
+        const bool rb = site_of(f.pattern, y, x) != 1;
+        s_lp[j][i] = (rb && inside(x, y, w, h, 2))
+                         ? stage_lpf([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; })
+                         : 0.0f;
+    }
+    __syncthreads();
+    // tile: CFA, VH, S4 PQ and S3 green to the planes
+    for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
+        const int j = k / TW, i = k - j * TW;
+        const int x = x0 + i, y = y0 + j;
+        if (x >= w || y >= h) continue;
+        const size_t o = (size_t)y * w + x;
+        const int cj = j + AC, ci = i + AC;
+        auto C = [&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; };
+        const bool rb = site_of(f.pattern, y, x) != 1;
+        const bool in4 = inside(x, y, w, h, 4);
+        P.cfa[o] = C(0, 0);
+        P.vh[o] = s_vh[j + 1][i + 1];
+        P.pq[o] = (rb && in4) ? stage_pq(C) : 0.5f;
+        float g;
+        if (!rb) g = C(0, 0);
+        else if (in4)
+            g = stage_s3(C, [&](int dx, int dy) { return s_vh[j + 1 + dy][i + 1 + dx]; },
+                         [&](int dx, int dy) { return s_lp[j + 2 + dy][i + 2 + dx]; });
+        else g = 0.0f;
+        P.c1[o] = g;
+    }
 }
 
-// S10 + S11: border fill (width 4) by source remapping, scale, round, clamp,
-// interleave (prism.cpp:302-313, common.hpp:98-129).
-__global__ __launch_bounds__(256) void k_prism_out(FrameParams f, void* ws) {
-    int x, y;
-    if (!xy_of(x, y, f.w, f.h)) return;
+// ------------------------------------------------------------------ pass B
+
+
+// pass B reads CFA with a 7-pixel apron, green 6, PQ 5, VH 2; chroma planes
+// cover the tile + 4
+__global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
+    __shared__ float s_cfa[Tile<7>::H][Tile<7>::W];
+    __shared__ float s_g[Tile<6>::H][Tile<6>::W];
+    __shared__ float s_pq[Tile<5>::H][Tile<5>::W];   // later: R after the median (tile)
+    __shared__ float s_vh[Tile<2>::H][Tile<2>::W];   // later: B after the median (tile)
+    __shared__ float s_r[Tile<4>::H][Tile<4>::W];    // chroma planes over tile + 4
+    __shared__ float s_b[Tile<4>::H][Tile<4>::W];
     const int w = f.w, h = f.h;
-    PrismPlanes P = planes_of(ws, w, h);
-    int sx, sy;
-    if (y < 4) { sy = 4; sx = x; }
-    else if (y >= h - 4) { sy = h - 5; sx = x; }
-    else { sy = y; sx = x < 4 ? 4 : (x >= w - 4 ? w - 5 : x); }
-    const size_t j = (size_t)sy * w + sx;
+    int x0, y0;
+    tile_origin(w, h, x0, y0);
+    const PrismPlanes P = planes_of(ws, w, h);
+
+    auto fill = [&](float* dst, int A, int RW, int RH, const float* src) {
+        for (int k = threadIdx.x; k < RW * RH; k += blockDim.x) {
+            const int j = k / RW, i = k - j * RW;
+            const int x = x0 - A + i, y = y0 - A + j;
+            dst[k] = (x >= 0 && x < w && y >= 0 && y < h) ? src[(size_t)y * w + x] : 0.0f;
+        }
+    };
+    fill(&s_cfa[0][0], 7, Tile<7>::W, Tile<7>::H, P.cfa);
+    fill(&s_g[0][0], 6, Tile<6>::W, Tile<6>::H, P.c1);
+    fill(&s_pq[0][0], 5, Tile<5>::W, Tile<5>::H, P.pq);
+    fill(&s_vh[0][0], 2, Tile<2>::W, Tile<2>::H, P.vh);
+    __syncthreads();
+
+    // chroma planes over tile + 4: natives, S5 at R/B sites
+    for (int k = threadIdx.x; k < Tile<4>::H * Tile<4>::W; k += blockDim.x) {
+        const int j = k / Tile<4>::W, i = k - j * Tile<4>::W;
+        const int x = x0 - 4 + i, y = y0 - 4 + j;
+        const int s = site_of(f.pattern, y, x);
+        const int cj = j + 3, ci = i + 3;  // s_cfa coordinates (apron 7)
+        const int gj = j + 2, gi = i + 2;  // s_g (apron 6)
+        const int qj = j + 1, qi = i + 1;  // s_pq (apron 5)
+        float r = 0.0f, b = 0.0f;
+        if (s != 1) {
+            const float nat = s_cfa[cj][ci];
+            float opp = 0.0f;
+            if (inside(x, y, w, h, 4))
+                opp = stage_s5([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; },
+                               [&](int dx, int dy) { return s_g[gj + dy][gi + dx]; },
+                               [&](int dx, int dy) { return s_pq[qj + dy][qi + dx]; });
+            if (s == 0) { r = nat; b = opp; } else { b = nat; r = opp; }
+        }
+        s_r[j][i] = r;
+        s_b[j][i] = b;
+    }
+    __syncthreads();
+    // S6 at G sites over tile + 1 (reads R/B sites only, writes G sites)
+    for (int k = threadIdx.x; k < Tile<1>::H * Tile<1>::W; k += blockDim.x) {
+        const int j = k / Tile<1>::W, i = k - j * Tile<1>::W;
+        const int x = x0 - 1 + i, y = y0 - 1 + j;
+        if (site_of(f.pattern, y, x) != 1 || !inside(x, y, w, h, 4)) continue;
+        const int rj = j + 3, ri = i + 3;  // chroma planes (apron 4)
+        const int gj = j + 5, gi = i + 5;  // s_g (apron 6)
+        const int vj = j + 1, vi = i + 1;  // s_vh (apron 2)
+        float vc = s_vh[vj][vi];
+        float vn = 0.25f * (s_vh[vj - 1][vi - 1] + s_vh[vj - 1][vi + 1] + s_vh[vj + 1][vi - 1] + s_vh[vj + 1][vi + 1]);
+        float disc = (fabsf(0.5f - vc) < fabsf(0.5f - vn)) ? vn : vc;
+        auto G = [&](int dx, int dy) { return s_g[gj + dy][gi + dx]; };
+        const float rv = stage_s6([&](int dx, int dy) { return s_r[rj + dy][ri + dx]; }, G, disc);
+        const float bv = stage_s6([&](int dx, int dy) { return s_b[rj + dy][ri + dx]; }, G, disc);
+        s_r[rj][ri] = rv;
+        s_b[rj][ri] = bv;
+    }
+    __syncthreads();
+    // S7-S9: medians of the colour differences over the tile; R into s_pq,
+    // B into s_vh (both dead now), tile coordinates
+    float* s_r9 = &s_pq[0][0];
+    float* s_b9 = &s_vh[0][0];
+    for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
+        const int j = k / TW, i = k - j * TW;
+        const int x = x0 + i, y = y0 + j;
+        const int rj = j + 4, ri = i + 4, gj = j + 6, gi = i + 6;
+        float r9 = s_r[rj][ri], b9 = s_b[rj][ri];
+        if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
+            float a[9], bb[9];
+            int q = 0;
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const float g = s_g[gj + dy][gi + dx];
+                    a[q] = s_r[rj + dy][ri + dx] - g;
+                    bb[q] = s_b[rj + dy][ri + dx] - g;
+                    ++q;
+                }
+            const float lm = median9(a), mm = median9(bb);
+            const float g0 = s_g[gj][gi];
+            r9 = clamp01(g0 + lm);
+            b9 = clamp01(g0 + mm);
+        }
+        s_r9[k] = r9;
+        s_b9[k] = b9;
+    }
+    __syncthreads();
+    // S10 + S11: width-4 border fill by source remapping (the source is in
+    // this tile), scale, round, clamp, interleave
     const float scale = (float)f.maxv;
-    int r = (int)fmaf(P.r9[j], scale, 0.5f);
-    int g = (int)fmaf(P.c1[j], scale, 0.5f);
-    int b = (int)fmaf(P.b9[j], scale, 0.5f);
-    r = iclamp(r, 0, f.maxv); g = iclamp(g, 0, f.maxv); b = iclamp(b, 0, f.maxv);
-    put_px(f, x, y, r, g, b);
+    for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
+        const int j = k / TW, i = k - j * TW;
+        const int x = x0 + i, y = y0 + j;
+        if (x >= w || y >= h) continue;
+        int sx, sy;
+        if (y < 4) { sy = 4; sx = x; }
+        else if (y >= h - 4) { sy = h - 5; sx = x; }
+        else { sy = y; sx = x < 4 ? 4 : (x >= w - 4 ? w - 5 : x); }
+        const int q = (sy - y0) * TW + (sx - x0);
+        int r = (int)fmaf(s_r9[q], scale, 0.5f);
+        int g = (int)fmaf(s_g[sy - y0 + 6][sx - x0 + 6], scale, 0.5f);
+        int b = (int)fmaf(s_b9[q], scale, 0.5f);
+        r = iclamp(r, 0, f.maxv); g = iclamp(g, 0, f.maxv); b = iclamp(b, 0, f.maxv);
+        put_px(f, x, y, r, g, b);
+    }
 }
 
 }  // namespace
 
-size_t prism_workspace_bytes(int w, int h) { return 9u * plane_stride(w, h) * sizeof(float); }
+size_t prism_workspace_bytes(int w, int h) { return 4u * plane_stride(w, h) * sizeof(float); }
 
 hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st) {
-    dim3 blk(64, 4);
-    dim3 grd((f.w + 63) / 64, (f.h + 3) / 4);
-    hipLaunchKernelGGL(k_prism_s0, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_dirs, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_s3, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_s5, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_s6, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_median, grd, blk, 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_out, grd, blk, 0, st, f, ws);
+    const dim3 grd((f.w + TW - 1) / TW, (f.h + TH - 1) / TH);
+    hipLaunchKernelGGL(k_prism_a, grd, dim3(256), 0, st, f, ws);
+    hipLaunchKernelGGL(k_prism_b, grd, dim3(256), 0, st, f, ws);
     return hipGetLastError();
 }
 
