@@ -140,6 +140,8 @@ hipError_t get_scratch(size_t need, void** out) {
 int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st) {
     hipError_t he;
     if (algo == kPrism) {
+        // the PRISM passes index their planes with 32-bit offsets
+        if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;
         void* ws = prism_ws;
         if (!ws) {
             he = get_scratch(prism_workspace_bytes(f.w, f.h), &ws);

@@ -69,6 +69,20 @@ __device__ __forceinline__ void tile_origin(int w, int h, int& x0, int& y0) {
     y0 = imin((int)blockIdx.y * TH, imax(h - TH, 0));
 }
 
+// k-th site of one class (rb: R/B sites, else G sites) in a region of RW
+// (even) x RH pixels whose top-left pixel is (xr, yr): every row holds RW / 2
+// of each class at alternating columns, so the threads of a wave all run the
+// same stage instead of half of them idling at the other class's sites.
+__device__ __forceinline__ void nth_site(int k, int RW, int xr, int yr, int pattern, bool rb, int& i, int& j) {
+    const int half = RW >> 1;
+    j = k / half;
+    const int m = k - j * half;
+    const int y = yr + j;
+    const int prb = ((y & 1) == r_row_of(pattern)) ? r_col_of(pattern) : 1 - r_col_of(pattern);
+    const int par = rb ? prb : 1 - prb;
+    i = 2 * m + ((par - xr) & 1);
+}
+
 // ------------------------------------------------------------ stage bodies
 // C(dx, dy): CFA at the pixel's offset; likewise the other accessors.
 
@@ -172,14 +186,17 @@ __device__ __forceinline__ float stage_s6(Xf X, Gf G, float disc) {
     return clamp01(fmaf(1.0f - disc, vEst, fmaf(disc, hEst, g0)));
 }
 
-// prism_detail::median_3x3 (prism.cpp:9-21): the same exchange network.
+// prism_detail::median_3x3 (prism.cpp:9-21): the same exchange network.  (It
+// is not a true median network -- it fails the 0-1 principle -- so it is kept
+// exactly.)  Each exchange "if (a > b) swap" is min / max: with no NaNs the
+// two differ only in which sign a +-0 tie keeps, and a signed zero selected
+// here leaves g + median, the clamp and the rounding unchanged.
 __device__ __forceinline__ float median9(float v[9]) {
 #define IMOP_CX(a, b)                        \
     {                                        \
-        float ta = v[a], tb = v[b];          \
-        bool sw = ta > tb;                   \
-        v[a] = sw ? tb : ta;                 \
-        v[b] = sw ? ta : tb;                 \
+        const float ta = v[a], tb = v[b];    \
+        v[a] = fminf(ta, tb);                \
+        v[b] = fmaxf(ta, tb);                \
     }
     IMOP_CX(0, 1) IMOP_CX(3, 4) IMOP_CX(6, 7) IMOP_CX(1, 2) IMOP_CX(4, 5) IMOP_CX(7, 8)
     IMOP_CX(0, 1) IMOP_CX(3, 4) IMOP_CX(6, 7) IMOP_CX(0, 3) IMOP_CX(4, 7) IMOP_CX(1, 4)
@@ -223,36 +240,44 @@ __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
         const int cj = j + AC - 1, ci = i + AC - 1;
         s_vh[j][i] = inside(x, y, w, h, 4) ? stage_vh([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; }) : 0.5f;
     }
-    for (int k = threadIdx.x; k < Tile<2>::H * Tile<2>::W; k += blockDim.x) {
-        const int j = k / Tile<2>::W, i = k - j * Tile<2>::W;
+    // (S3 reads the low-pass only at R/B sites)
+    for (int k = threadIdx.x; k < Tile<2>::H * Tile<2>::W / 2; k += blockDim.x) {
+        int i, j;
+        nth_site(k, Tile<2>::W, x0 - 2, y0 - 2, f.pattern, true, i, j);
         const int x = x0 - 2 + i, y = y0 - 2 + j;
         const int cj = j + AC - 2, ci = i + AC - 2;
-        const bool rb = site_of(f.pattern, y, x) != 1;
-        s_lp[j][i] = (rb && inside(x, y, w, h, 2))
-                         ? stage_lpf([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; })
-                         : 0.0f;
+        s_lp[j][i] = inside(x, y, w, h, 2) ? stage_lpf([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; })
+                                           : 0.0f;
     }
     __syncthreads();
-    // tile: CFA, VH, S4 PQ and S3 green to the planes
+    // tile: CFA and VH everywhere; at G sites green = CFA and PQ = 0.5
     for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
         const int j = k / TW, i = k - j * TW;
         const int x = x0 + i, y = y0 + j;
         if (x >= w || y >= h) continue;
         const size_t o = (size_t)y * w + x;
+        const float c = s_cfa[j + AC][i + AC];
+        P.cfa[o] = c;
+        P.vh[o] = s_vh[j + 1][i + 1];
+        if (site_of(f.pattern, y, x) == 1) {
+            P.pq[o] = 0.5f;
+            P.c1[o] = c;
+        }
+    }
+    // tile R/B sites: S4 PQ and S3 green
+    for (int k = threadIdx.x; k < TH * TW / 2; k += blockDim.x) {
+        int i, j;
+        nth_site(k, TW, x0, y0, f.pattern, true, i, j);
+        const int x = x0 + i, y = y0 + j;
+        if (x >= w || y >= h) continue;
+        const size_t o = (size_t)y * w + x;
         const int cj = j + AC, ci = i + AC;
         auto C = [&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; };
-        const bool rb = site_of(f.pattern, y, x) != 1;
         const bool in4 = inside(x, y, w, h, 4);
-        P.cfa[o] = C(0, 0);
-        P.vh[o] = s_vh[j + 1][i + 1];
-        P.pq[o] = (rb && in4) ? stage_pq(C) : 0.5f;
-        float g;
-        if (!rb) g = C(0, 0);
-        else if (in4)
-            g = stage_s3(C, [&](int dx, int dy) { return s_vh[j + 1 + dy][i + 1 + dx]; },
-                         [&](int dx, int dy) { return s_lp[j + 2 + dy][i + 2 + dx]; });
-        else g = 0.0f;
-        P.c1[o] = g;
+        P.pq[o] = in4 ? stage_pq(C) : 0.5f;
+        P.c1[o] = in4 ? stage_s3(C, [&](int dx, int dy) { return s_vh[j + 1 + dy][i + 1 + dx]; },
+                                 [&](int dx, int dy) { return s_lp[j + 2 + dy][i + 2 + dx]; })
+                      : 0.0f;
     }
 }
 
@@ -273,11 +298,13 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
     tile_origin(w, h, x0, y0);
     const PrismPlanes P = planes_of(ws, w, h);
 
+    // (frames on this path have fewer than 2^31 pixels: 32-bit offsets)
     auto fill = [&](float* dst, int A, int RW, int RH, const float* src) {
         for (int k = threadIdx.x; k < RW * RH; k += blockDim.x) {
             const int j = k / RW, i = k - j * RW;
             const int x = x0 - A + i, y = y0 - A + j;
-            dst[k] = (x >= 0 && x < w && y >= 0 && y < h) ? src[(size_t)y * w + x] : 0.0f;
+            const bool ok = (unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h;
+            dst[k] = ok ? src[(uint32_t)(y * w + x)] : 0.0f;
         }
     };
     fill(&s_cfa[0][0], 7, Tile<7>::W, Tile<7>::H, P.cfa);
@@ -286,33 +313,38 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
     fill(&s_vh[0][0], 2, Tile<2>::W, Tile<2>::H, P.vh);
     __syncthreads();
 
-    // chroma planes over tile + 4: natives, S5 at R/B sites
-    for (int k = threadIdx.x; k < Tile<4>::H * Tile<4>::W; k += blockDim.x) {
-        const int j = k / Tile<4>::W, i = k - j * Tile<4>::W;
+    // chroma planes over tile + 4: at R/B sites the native and S5's opposite
+    // chroma, at G sites 0 until S6 (the two loops write disjoint entries)
+    for (int k = threadIdx.x; k < Tile<4>::H * Tile<4>::W / 2; k += blockDim.x) {
+        int i, j;
+        nth_site(k, Tile<4>::W, x0 - 4, y0 - 4, f.pattern, true, i, j);
         const int x = x0 - 4 + i, y = y0 - 4 + j;
-        const int s = site_of(f.pattern, y, x);
         const int cj = j + 3, ci = i + 3;  // s_cfa coordinates (apron 7)
         const int gj = j + 2, gi = i + 2;  // s_g (apron 6)
         const int qj = j + 1, qi = i + 1;  // s_pq (apron 5)
-        float r = 0.0f, b = 0.0f;
-        if (s != 1) {
-            const float nat = s_cfa[cj][ci];
-            float opp = 0.0f;
-            if (inside(x, y, w, h, 4))
-                opp = stage_s5([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; },
-                               [&](int dx, int dy) { return s_g[gj + dy][gi + dx]; },
-                               [&](int dx, int dy) { return s_pq[qj + dy][qi + dx]; });
-            if (s == 0) { r = nat; b = opp; } else { b = nat; r = opp; }
-        }
-        s_r[j][i] = r;
-        s_b[j][i] = b;
+        const float nat = s_cfa[cj][ci];
+        float opp = 0.0f;
+        if (inside(x, y, w, h, 4))
+            opp = stage_s5([&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; },
+                           [&](int dx, int dy) { return s_g[gj + dy][gi + dx]; },
+                           [&](int dx, int dy) { return s_pq[qj + dy][qi + dx]; });
+        const bool red = site_of(f.pattern, y, x) == 0;
+        s_r[j][i] = red ? nat : opp;
+        s_b[j][i] = red ? opp : nat;
+    }
+    for (int k = threadIdx.x; k < Tile<4>::H * Tile<4>::W / 2; k += blockDim.x) {
+        int i, j;
+        nth_site(k, Tile<4>::W, x0 - 4, y0 - 4, f.pattern, false, i, j);
+        s_r[j][i] = 0.0f;
+        s_b[j][i] = 0.0f;
     }
     __syncthreads();
     // S6 at G sites over tile + 1 (reads R/B sites only, writes G sites)
-    for (int k = threadIdx.x; k < Tile<1>::H * Tile<1>::W; k += blockDim.x) {
-        const int j = k / Tile<1>::W, i = k - j * Tile<1>::W;
+    for (int k = threadIdx.x; k < Tile<1>::H * Tile<1>::W / 2; k += blockDim.x) {
+        int i, j;
+        nth_site(k, Tile<1>::W, x0 - 1, y0 - 1, f.pattern, false, i, j);
         const int x = x0 - 1 + i, y = y0 - 1 + j;
-        if (site_of(f.pattern, y, x) != 1 || !inside(x, y, w, h, 4)) continue;
+        if (!inside(x, y, w, h, 4)) continue;
         const int rj = j + 3, ri = i + 3;  // chroma planes (apron 4)
         const int gj = j + 5, gi = i + 5;  // s_g (apron 6)
         const int vj = j + 1, vi = i + 1;  // s_vh (apron 2)
