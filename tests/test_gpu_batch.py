@@ -183,3 +183,26 @@ def test_invalid_frame_enqueues_nothing():
     plan = imop_amd.BatchPlan(descs)
     assert (plan.error, plan.failed_frame) == (int(imop_amd.DemosaicError.InternalError), 2)
     assert imop_amd.demosaic_batch([]) == (0, -1)
+
+
+@pytest.mark.parametrize("w,h,bd,algo", [(1006, 758, 8, MG), (640, 480, 12, HQLI), (131, 67, 10, PRISM),
+                                         (4024, 3032, 12, MG)])
+def test_cpp_demosaic_cuda_batch_matches_per_frame(w, h, bd, algo):
+    """imop::demosaic_cuda_batch (the reference's C++ entry point, host
+    buffers) equals imop::demosaic_cuda frame by frame."""
+    import ctypes
+    fn = getattr(imop_amd.lib(), "_ZN4imop19demosaic_cuda_batchEPKPKhPKPhiiiNS_12BayerPatternENS_17DemosaicAlgorithmEi")
+    fn.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)] + [ctypes.c_int] * 6
+    fn.restype = ctypes.c_int
+    n = 3 if w * h > 1_000_000 else 5
+    rng = np.random.default_rng(w + bd)
+    raws = [rng.integers(0, 1 << bd, size=(h, w)).astype(np.uint8 if bd <= 8 else np.uint16) for _ in range(n)]
+    outs = [np.zeros(imop_amd.rgb_byte_size(w, h, bd), dtype=np.uint8) for _ in range(n)]
+    ins = (ctypes.c_void_p * n)(*[r.ctypes.data for r in raws])
+    ops = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    print(f"demosaic_cuda_batch {w}x{h} bd{bd} algo {int(algo)}", flush=True)
+    assert fn(ins, ops, n, w, h, 1, int(algo), bd) == 0
+    for r, o in zip(raws, outs):
+        e, want = imop_amd.demosaic_host(r, w, h, 1, algo, bd)
+        assert e == 0
+        np.testing.assert_array_equal(o.view(np.uint16 if bd > 8 else np.uint8).reshape(h, w, 3), want)
