@@ -8,7 +8,7 @@
 //              colour-difference medians (S7-S9), the width-4 border fill,
 //              rounding and the interleaved RGB write (S10, S11).
 //
-// Each workgroup owns a 64 x 16 tile of output pixels and computes every
+// Each workgroup owns a 32 x 16 tile of output pixels and computes every
 // intermediate it needs over the tile plus an apron in LDS: S3 reads CFA up
 // to 5 pixels away (VH at +-1 of a +-4 window), and S5 -> S6 -> median reach
 // 7 pixels into the four planes.  The last tile of each row / column is
@@ -37,7 +37,7 @@ namespace {
 constexpr float kEps = 1e-5f;
 constexpr float kEpsSq = 1e-10f;
 
-constexpr int TW = 64, TH = 16;  // output tile
+constexpr int TW = 32, TH = 16;  // output tile (measured best of 128x16, 64x16, 64x8, 32x32, 32x16)
 
 struct PrismPlanes {
     float *cfa, *c1, *vh, *pq;
