@@ -63,10 +63,19 @@ __device__ __forceinline__ bool inside(int x, int y, int w, int h, int b) {
     return y >= b && y < h - b && x >= b && x < w - b;
 }
 
-// Tile origin; the last tile of a row / column is shifted inward.
+// Tile of this workgroup and its origin.  The 1D grid is remapped so that
+// each of the 8 XCDs walks a contiguous raster run of tiles (workgroups are
+// dealt round-robin to the XCDs; with nblk = 8q + r, XCD x runs q + (x < r)
+// of them): neighbouring tiles, which re-read each other's aprons, then share
+// one L2.  The last tile of a row / column is shifted inward.
 __device__ __forceinline__ void tile_origin(int w, int h, int& x0, int& y0) {
-    x0 = imin((int)blockIdx.x * TW, imax(w - TW, 0));
-    y0 = imin((int)blockIdx.y * TH, imax(h - TH, 0));
+    const int nblk = gridDim.x;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int t = xcd * (nblk >> 3) + imin(xcd, nblk & 7) + slot;
+    const int gx = (w + TW - 1) / TW;
+    const int ty = t / gx, tx = t - ty * gx;
+    x0 = imin(tx * TW, imax(w - TW, 0));
+    y0 = imin(ty * TH, imax(h - TH, 0));
 }
 
 // k-th site of one class (rb: R/B sites, else G sites) in a region of RW
@@ -413,7 +422,7 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
 size_t prism_workspace_bytes(int w, int h) { return 4u * plane_stride(w, h) * sizeof(float); }
 
 hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st) {
-    const dim3 grd((f.w + TW - 1) / TW, (f.h + TH - 1) / TH);
+    const dim3 grd(((f.w + TW - 1) / TW) * ((f.h + TH - 1) / TH));
     hipLaunchKernelGGL(k_prism_a, grd, dim3(256), 0, st, f, ws);
     hipLaunchKernelGGL(k_prism_b, grd, dim3(256), 0, st, f, ws);
     return hipGetLastError();
