@@ -20,7 +20,7 @@ struct BatchFrame {
     int32_t fmt;           // FMT_U8 / FMT_U16 / FMT_P12 / FMT_P10
     int32_t flags;         // bit 0: RGB as u16; bit 1: black level applied
     int32_t bl_i[4];       // integral black level by (y & 1) * 2 + (x & 1); 0 without
-    uint16_t nsx, nby;     // strips (248 columns) and bands at the batch's rows per wave
+    uint16_t nsx, nby;     // strips (strip_count) and bands at the batch's rows per wave
     uint32_t first;        // index of the frame's first work item in the batch
 };
 static_assert(sizeof(BatchFrame) == 72, "BatchFrame layout");

@@ -41,6 +41,20 @@ struct FrameParams {
     float bl_q[4];        // offset by (y & 1) * 2 + (x & 1), already max(0, .)
 };
 
+// Strips of the register-resident kernel.  A wave computes 248 columns
+// [xs, xs + 248), xs = stride * strip.  u8 output: stride 248, each strip
+// writes its own columns.  u16 output: stride 224, and in row y the strip
+// boundaries sit at stride * k + s(y), s(y) in {0, 8, 16, 24} chosen so that
+// each boundary's output byte is 64-byte aligned; every write window then
+// lies inside its wave's computed columns, and no 64-byte sector of the
+// output is written half by one wave and half by another (such sectors
+// stream at ~4.1 TB/s against ~5 TB/s for whole sectors).
+__host__ __device__ inline int strip_stride(int out16) { return out16 ? 224 : 248; }
+__host__ __device__ inline int strip_count(int w, int out16) {
+    const int s = strip_stride(out16);
+    return w <= 248 ? 1 : (w - 248 + s - 1) / s + 1;
+}
+
 // Parameters of the register-resident HQLI / MG kernel (fast_kernels.hip).
 struct FastParams {
     const uint8_t* in;

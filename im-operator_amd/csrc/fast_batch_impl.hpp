@@ -2,7 +2,7 @@
 // HQLI / MG kernel (fast_impl.hpp).
 //
 // Work list.  Every frame of the batch is cut into the same work items as a
-// single-frame launch: one wave-sized strip of 248 columns by a band of TH
+// single-frame launch: one wave-sized strip (strip_count) by a band of TH
 // rows.  Items are numbered frame by frame (BatchFrame::first), so the list
 // is ragged: frames of any size, layout (u8, u16, 12-bit, 10-bit packed),
 // width class (w % 4 == 0 or 2), bit depth, black level and output width
@@ -25,19 +25,19 @@ namespace imop_hip {
 namespace {
 
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
-__device__ __forceinline__ void batch_item(const FastParams& f, int x, int y0, bool lane_out, bool careful,
+__device__ __forceinline__ void batch_item(const FastParams& f, const Strip& sp, int y0, bool careful,
                                            uint32_t* stage) {
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out, stage);
-    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out, stage);
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, sp, y0, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, sp, y0, stage);
 }
 
 template <int FMT, int PAT, int ALG, int TH>
-__device__ __forceinline__ void batch_fmt(const FastParams& f, int x, int y0, bool lane_out, bool careful,
+__device__ __forceinline__ void batch_fmt(const FastParams& f, const Strip& sp, int y0, bool careful,
                                           uint32_t* stage) {
     // MG: 8-bit frames take the reference's integer path, deeper ones the exact rational one
     constexpr int MODE = ALG == 0 ? MODE_HQLI : (FMT == FMT_U8 ? MODE_MG_INT8 : MODE_MG_EXACT);
-    if (f.w & 2) batch_item<FMT, PAT, MODE, TH, true>(f, x, y0, lane_out, careful, stage);
-    else batch_item<FMT, PAT, MODE, TH, false>(f, x, y0, lane_out, careful, stage);
+    if (f.w & 2) batch_item<FMT, PAT, MODE, TH, true>(f, sp, y0, careful, stage);
+    else batch_item<FMT, PAT, MODE, TH, false>(f, sp, y0, careful, stage);
 }
 
 }  // namespace
@@ -46,7 +46,6 @@ template <int PAT, int ALG, int TH>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
     __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
     uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
-    const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // S slices, one per XCD (fewer when the grid has fewer than 8
     // workgroups, so that every slice has at least one workgroup)
@@ -92,16 +91,14 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         const bool strip_fastest = (a.order & 2) != 0;
         const int by = strip_fastest ? local / nsx : local % nby;
         const int sx = strip_fastest ? local % nsx : local / nby;
-        const int x = sx * 248 - 4 + 4 * lane;
+        const Strip sp = strip_of(f, sx);
         const int y0 = by * TH;
-        const bool lane_out = lane >= 1 && lane <= 62 && x < f.w;
-        const int xs0 = sx * 248, xs1 = xs0 + 248;
-        const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
+        const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
         switch (bf.fmt) {
-        case FMT_U8: batch_fmt<FMT_U8, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
-        case FMT_U16: batch_fmt<FMT_U16, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
-        case FMT_P12: batch_fmt<FMT_P12, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
-        default: batch_fmt<FMT_P10, PAT, ALG, TH>(f, x, y0, lane_out, careful, stage); break;
+        case FMT_U8: batch_fmt<FMT_U8, PAT, ALG, TH>(f, sp, y0, careful, stage); break;
+        case FMT_U16: batch_fmt<FMT_U16, PAT, ALG, TH>(f, sp, y0, careful, stage); break;
+        case FMT_P12: batch_fmt<FMT_P12, PAT, ALG, TH>(f, sp, y0, careful, stage); break;
+        default: batch_fmt<FMT_P10, PAT, ALG, TH>(f, sp, y0, careful, stage); break;
         }
     }
 }

@@ -2,10 +2,13 @@
 // HQLI / MG demosaic for gfx950.
 //
 // Geometry.  A wave owns a strip of 64 lanes x 4 pixels = 256 columns and a
-// band of TH rows.  Lanes 1..62 produce output (248 columns); lanes 0 and 63
-// only load, so that every producing lane finds its horizontal neighbours in
-// the adjacent lanes (DPP wave_shr / wave_shl, one VALU op per value).  Strips
-// overlap by 8 columns; consecutive strips' producing lanes tile a row.
+// band of TH rows.  Lanes 1..62 compute 248 columns; lanes 0 and 63 only
+// load, so that every computing lane finds its horizontal neighbours in the
+// adjacent lanes (DPP wave_shr / wave_shl, one VALU op per value).  Strips
+// advance 248 columns for u8 output; for u16 output they advance 224 and each
+// row's strip boundaries shift so that they fall on 64-byte boundaries of the
+// output (strip_stride, row_shift), each wave writing a window of the columns
+// it computed.
 //
 // Each lane walks down its band two rows (one CFA quad row) per step, keeping
 // a sliding window of 6 raw rows and 4 rows of MG colour differences in
@@ -283,11 +286,54 @@ __device__ __forceinline__ void store_chunk(uint8_t* base, uint32_t off, uint4 v
     }
 }
 
-// wr: this lane holds output pixels of the row; nbytes: the segment's length
-// (248 columns, fewer at the frame's right edge).
-__device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* stage, int lane, uint32_t seg_px,
-                                                 const int r[4], const int g[4], const int b[4], bool wr,
-                                                 uint32_t nbytes) {
+// Strip geometry (strip_stride / strip_count): the wave's computed columns
+// start at xs (lane run x = xs - 4 + 4 * lane, lanes 1..62 produce), and in
+// row y it writes columns [lo, hi) (seg_cols).
+struct Strip {
+    int xs;
+    bool first, last;
+};
+
+__device__ __forceinline__ Strip strip_of(const FastParams& f, int sx) {
+    return Strip{sx * strip_stride(f.out16), sx == 0, sx == f.nsx - 1};
+}
+
+// s(y): u16 output, the shift in {0, 8, 16, 24} that puts the row's strip
+// boundaries on 64-byte boundaries (48 * s/8 == -(out + 6 * y * w) mod 64
+// needs that address to be a multiple of 16; otherwise 0).  Only the low
+// bits of the address matter, so 32-bit wrap-around is harmless.
+__device__ __forceinline__ int row_shift(const FastParams& f, int y) {
+    if (!f.out16) return 0;
+    const uint32_t a = (uint32_t)(uintptr_t)f.out + 6u * (uint32_t)y * (uint32_t)f.w;
+    if (a & 15u) return 0;
+    const uint32_t t = ((64u - (a & 63u)) & 63u) >> 4;  // -a / 16 (mod 4)
+    return 8 * (int)((3u * t) & 3u);                     // 3 * 3 == 1 (mod 4)
+}
+
+__device__ __forceinline__ void seg_cols(const FastParams& f, const Strip& sp, int y, int& lo, int& hi) {
+    const int sh = row_shift(f, y);
+    lo = sp.first ? 0 : sp.xs + sh;
+    hi = sp.last ? f.w : imin(sp.xs + strip_stride(f.out16) + sh, f.w);
+}
+
+// This lane's 4 pixels at x lie in the row's write window (windows start at
+// multiples of 8 columns, so a lane run is wholly in or out).
+__device__ __forceinline__ bool lane_writes(const FastParams& f, const Strip& sp, int x, int y) {
+    const int lane = threadIdx.x & 63;
+    int lo, hi;
+    seg_cols(f, sp, y, lo, hi);
+    return lane >= 1 && lane <= 62 && x >= lo && x < hi;
+}
+
+// The staged row holds the producing lanes' 248 pixels from column xs (lane
+// l at pixel 4 * (l - 1)); columns [c0, c1) of it (relative to xs) are
+// written at row pixel index row_px = y * w + xs.  c0 * (bytes per pixel) is
+// a multiple of 16 (u16: c0 in {0, 8, 16, 24}; u8: c0 == 0), so the window
+// is read from LDS in whole 16-byte chunks, and no chunk read lies past the
+// staged 1488 bytes by more than one chunk (the slice holds 1536).
+__device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* stage, int lane, uint32_t row_px,
+                                                 int c0, int c1, const int r[4], const int g[4], const int b[4],
+                                                 bool wr) {
     if (f.out16) {
         if (wr) {
             uint2* p = (uint2*)(stage + 6 * (lane - 1));
@@ -303,14 +349,18 @@ __device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* 
             p[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
         }
     }
+    const uint32_t e = f.out16 ? 6u : 3u;
+    const uint32_t b0 = e * (uint32_t)c0, nbytes = e * (uint32_t)(c1 - c0);
+    const uint32_t i0 = b0 >> 4;
+    const bool two = lane < 32 && 16u * (64u + (uint32_t)lane) < nbytes;
     lds_order();
-    const uint4 v0 = ((const uint4*)stage)[lane];
+    const uint4 v0 = ((const uint4*)stage)[i0 + (uint32_t)lane];
     uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
-    if (lane < 32) v1 = ((const uint4*)stage)[64 + lane];
+    if (two) v1 = ((const uint4*)stage)[i0 + 64u + (uint32_t)lane];
     lds_order();
-    uint8_t* base = f.out + (f.out16 ? 6u : 3u) * seg_px;
+    uint8_t* base = f.out + e * (row_px + (uint32_t)c0);
     store_chunk(base, 16u * (uint32_t)lane, v0, nbytes);
-    if (lane < 32) store_chunk(base, 1024u + 16u * (uint32_t)lane, v1, nbytes);
+    if (two) store_chunk(base, 1024u + 16u * (uint32_t)lane, v1, nbytes);
 }
 
 // MG colour difference of the two R/B sites a lane owns in one row.
@@ -396,173 +446,203 @@ __device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const 
     }
 }
 
-// One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
-template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD>
-__device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, bool lane_out, uint32_t* stage) {
-    constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
-    const int w = f.w, h = f.h;
-    const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
-    const bool lane_ok = x >= 0 && x < w;
-    const uint32_t xu = lane_ok ? (uint32_t)x : 0u;
-    bool col_ok[4], col_core[4];
+// Raw bytes of the lane run in row y.  Interior bands: every row and lane
+// is inside the frame, so no masking; their 8-byte windows end
+// before the stream does because the final row is never in their window.
+template <int FMT, bool CAREFUL, bool ODD>
+__device__ __forceinline__ Raw load_row(const FastParams& f, int y, uint32_t xu, bool lane_ok, uint32_t rs,
+                                        uint32_t lo) {
+    constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10;
+    const bool row_ok = !CAREFUL || (y >= 0 && y < f.h);
+    if (ODD) {
+        // the last row is decoded pixel by pixel (its final group may be
+        // cut short); every other row has >= 11 stream bytes past the run
+        const bool ok = row_ok && lane_ok && !(CAREFUL && y == f.h - 1);
+        return load_raw_any<FMT>(f, ok ? run_offset<FMT>((uint32_t)f.w, xu, (uint32_t)y) : 0u);
+    }
+    const uint32_t off = (row_ok ? (uint32_t)y : 0u) * rs + lo;
+    if (CAREFUL && packed) return load_raw_exact<FMT>(f, off, row_ok && lane_ok);
+    return load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
+}
+
+// Decoded, black-levelled samples of row y (parity YODD) from its raw bytes.
+template <int FMT, bool CAREFUL, bool ODD, int YODD>
+__device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Raw raw, bool lane_ok, uint32_t xu,
+                                           int P[4]) {
+    const int o0 = f.bl_i[YODD * 2], o1 = f.bl_i[YODD * 2 + 1];
+    const bool row_ok = y >= 0 && y < f.h;
+    uint32_t v[4];
+    if (ODD && CAREFUL && y == f.h - 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < f.w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
+    } else {
+        raw_values<FMT>(raw, ODD && YODD, v);  // w % 4 == 2: odd rows start at sub-pixel 2
+    }
+    finish4<FMT, CAREFUL>(f, v, row_ok && lane_ok, o0, o1, P);
+}
+
+// Interior site columns of the lane run (MG green kernel) and the columns the
+// colour-difference pass reaches.
+__device__ __forceinline__ void lane_columns(int x, int w, bool col_ok[4], bool col_core[4]) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        col_ok[k] = (x + k >= 2) && (x + k < w - 2);     // interior site column (MG green kernel)
-        col_core[k] = (x + k >= 1) && (x + k < w - 1);   // computed by the colour-difference pass
+        col_ok[k] = (x + k >= 2) && (x + k < w - 2);
+        col_core[k] = (x + k >= 1) && (x + k < w - 1);
     }
-    constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10;
+}
 
-    // issue every load of the band (interior bands: every row and lane is
-    // inside the frame, so no masking; their 8-byte windows end before the
-    // stream does because the final row is never in their window)
-    const uint32_t rs = row_stride<FMT>((uint32_t)w), lo = lane_offset<FMT>(xu);
-    Raw raw[NR];
+// One step: output rows yA, yA + 1 (yA even) of the lane run at x.  Pw holds
+// the decoded rows yA-1 .. yA+4; Dw the MG colour differences of rows yA-1,
+// yA on entry, and the step forms rows yA+1, yA+2 into Dw[2], Dw[3].
+template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD>
+__device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, int x, int yA, uint32_t* stage,
+                                          const int (*Pw)[4], int (*Dw)[3], const bool col_ok[4],
+                                          const bool col_core[4]) {
+    const int w = f.w, h = f.h;
+    const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
+    if (MODE != MODE_HQLI) {
+        d_row<PAT, MODE, CAREFUL>(&Pw[0], 1, (yA + 1 >= 2) && (yA + 1 < h - 2), col_ok, Dw[2]);
+        d_row<PAT, MODE, CAREFUL>(&Pw[1], 0, (yA + 2 >= 2) && (yA + 2 < h - 2), col_ok, Dw[3]);
+    }
+    int R[2][4], G[2][4], B[2][4];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-        const int y = y0 - 3 + i;
-        const bool row_ok = !CAREFUL || (y >= 0 && y < h);
-        if (ODD) {
-            // the last row is decoded pixel by pixel (its final group may be
-            // cut short); every other row has >= 11 stream bytes past the run
-            const bool ok = row_ok && lane_ok && !(CAREFUL && y == h - 1);
-            raw[i] = load_raw_any<FMT>(f, ok ? run_offset<FMT>((uint32_t)w, xu, (uint32_t)y) : 0u);
+    for (int a = 0; a < 2; ++a) {
+        const int* p = Pw[1 + a];
+        const int* pu = Pw[a];
+        const int* pd = Pw[2 + a];
+        int hl = 0, hr = 0, ul = 0, ur = 0, dl = 0, dr = 0;
+        if (MODE == MODE_HQLI) {
+            hl = from_left(p[3]); hr = from_right(p[0]);
+            ul = from_left(pu[3]); ur = from_right(pu[0]);
+            dl = from_left(pd[3]); dr = from_right(pd[0]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = p[k];
+            int r, g, b;
+            if (MODE == MODE_HQLI) {
+                const int s = role<PAT>(a, k & 1);
+                const int le = k > 0 ? p[k - 1] : hl, ri = k < 3 ? p[k + 1] : hr;
+                const int up = pu[k], dn = pd[k];
+                if (s != 1) {
+                    const int ul_ = k > 0 ? pu[k - 1] : ul, ur_ = k < 3 ? pu[k + 1] : ur;
+                    const int dl_ = k > 0 ? pd[k - 1] : dl, dr_ = k < 3 ? pd[k + 1] : dr;
+                    const int gg = (le + ri + up + dn + 2) >> 2;
+                    const int oth = (ul_ + ur_ + dl_ + dr_ + 2) >> 2;
+                    if (s == 0) { r = c; g = gg; b = oth; } else { b = c; g = gg; r = oth; }
+                } else {
+                    const int hz = (le + ri + 1) >> 1, vt = (up + dn + 1) >> 1;
+                    g = c;
+                    if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
+                }
+                r = imin(r, ch); g = imin(g, ch); b = imin(b, ch);
+            } else {
+                mg_px<PAT, MODE>(a, k, c, Dw[a], Dw[a + 1], Dw[a + 2], ch, r, g, b);
+                if (CAREFUL) {
+                    // positions the colour-difference pass never reaches keep R = B = 0
+                    const bool core = col_core[k] && (yA + a >= 1) && (yA + a < h - 1);
+                    r = core ? r : 0;
+                    b = core ? b : 0;
+                }
+            }
+            R[a][k] = r; G[a][k] = g; B[a][k] = b;
+        }
+    }
+    // w % 4 == 2: column w-3 of the right-edge column fill lives in the
+    // lane to the left (DPP while every lane is active)
+    int LR[2] = {0, 0}, LG[2] = {0, 0}, LB[2] = {0, 0};
+    if (ODD && CAREFUL) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            LR[a] = from_left(R[a][3]);
+            LG[a] = from_left(G[a][3]);
+            LB[a] = from_left(B[a][3]);
+        }
+    }
+
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int y = yA + a;
+        const int lane = threadIdx.x & 63;
+        if (!CAREFUL) {
+            // staged rows need 4-byte aligned runs: always for u16, for u8 when w % 4 == 0
+            if (!ODD || f.out16) {
+                int lo, hi;
+                seg_cols(f, sp, y, lo, hi);
+                store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)sp.xs, lo - sp.xs,
+                                 hi - sp.xs, R[a], G[a], B[a], lane >= 1 && lane <= 62);
+            } else if (lane_writes(f, sp, x, y)) {
+                store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
+            }
             continue;
         }
-        const uint32_t off = (row_ok ? (uint32_t)y : 0u) * rs + lo;
-        if (CAREFUL && packed) raw[i] = load_raw_exact<FMT>(f, off, row_ok && lane_ok);
-        else raw[i] = load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
-    }
-
-    int P[NR][4];
-    auto decode_row = [&](int i) {
-        const int y = y0 - 3 + i;
-        const int o0 = f.bl_i[(y & 1) * 2], o1 = f.bl_i[(y & 1) * 2 + 1];
-        const bool row_ok = y >= 0 && y < h;
-        uint32_t v[4];
-        if (ODD && CAREFUL && y == h - 1) {
+        if (y >= h || y < 2 || y >= h - 2) continue;  // the same for every lane
+        int r[4], g[4], b[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
-        } else {
-            raw_values<FMT>(raw[i], ODD && ((i + 1) & 1), v);  // y0 even: row y is odd when i is even
+        for (int k = 0; k < 4; ++k) { r[k] = R[a][k]; g[k] = G[a][k]; b[k] = B[a][k]; }
+        if (MODE == MODE_HQLI) {
+            // pre-fill value of a border column: never written -> 0
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (!col_ok[k]) { r[k] = 0; g[k] = 0; b[k] = 0; }
         }
-        finish4<FMT, CAREFUL>(f, v, row_ok && lane_ok, o0, o1, P[i]);
-    };
+        if (y == 2) {
+            if (lane_writes(f, sp, x, 0)) store4<ODD>(f, x, 0, r, g, b);
+            if (lane_writes(f, sp, x, 1)) store4<ODD>(f, x, 1, r, g, b);
+        }
+        if (y == h - 3) {
+            if (lane_writes(f, sp, x, h - 2)) store4<ODD>(f, x, h - 2, r, g, b);
+            if (lane_writes(f, sp, x, h - 1)) store4<ODD>(f, x, h - 1, r, g, b);
+        }
+        // column fill of the middle rows: columns 0-1 <- 2, w-2..w-1 <- w-3
+        if (x == 0) { r[0] = r[1] = R[a][2]; g[0] = g[1] = G[a][2]; b[0] = b[1] = B[a][2]; }
+        if (!ODD && x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
+        if (ODD && x == w - 2) { r[0] = r[1] = LR[a]; g[0] = g[1] = LG[a]; b[0] = b[1] = LB[a]; }
+        if (!ODD || f.out16) {
+            // staged like the interior rows; the window ends at the frame's right edge
+            int lo, hi;
+            seg_cols(f, sp, y, lo, hi);
+            store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)sp.xs, lo - sp.xs, hi - sp.xs,
+                             r, g, b, lane >= 1 && lane <= 62 && x < w);
+        } else if (lane_writes(f, sp, x, y)) {
+            store4<ODD>(f, x, y, r, g, b);
+        }
+    }
+}
 
-    int D[TH + 2][3];  // MG colour differences, rows y0-1 .. y0+TH
-    auto d_row_at = [&](int j) {  // D row j <-> image row y0 - 1 + j, window centre P[j + 2]
-        const int y = y0 - 1 + j;
-        d_row<PAT, MODE, CAREFUL>(&P[j], (j + 1) & 1, (y >= 2) && (y < h - 2), col_ok, D[j]);
-    };
-
+// One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
+// Every load of the band is issued up front.
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD>
+__device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage) {
+    constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
+    const int x = sp.xs - 4 + 4 * (int)(threadIdx.x & 63);
+    const bool lane_ok = x >= 0 && x < f.w;
+    const uint32_t xu = lane_ok ? (uint32_t)x : 0u;
+    bool col_ok[4], col_core[4];
+    lane_columns(x, f.w, col_ok, col_core);
+    const uint32_t rs = row_stride<FMT>((uint32_t)f.w), lo = lane_offset<FMT>(xu);
+    Raw raw[NR];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) decode_row(i);
-    if (MODE != MODE_HQLI) { d_row_at(0); d_row_at(1); }
+    for (int i = 0; i < NR; ++i) raw[i] = load_row<FMT, CAREFUL, ODD>(f, y0 - 3 + i, xu, lane_ok, rs, lo);
 
+    int P[NR][4];  // row y0 - 3 + i; y0 even, so row i is odd when i is even
+    auto decode = [&](int i) {
+        if (i & 1) decode_row<FMT, CAREFUL, ODD, 0>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+        else decode_row<FMT, CAREFUL, ODD, 1>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+    };
+#pragma unroll
+    for (int i = 0; i < 6; ++i) decode(i);
+    int D[TH + 2][3];  // MG colour differences, rows y0-1 .. y0+TH
+    if (MODE != MODE_HQLI) {
+        d_row<PAT, MODE, CAREFUL>(&P[0], 1, (y0 - 1 >= 2) && (y0 - 1 < f.h - 2), col_ok, D[0]);
+        d_row<PAT, MODE, CAREFUL>(&P[1], 0, (y0 >= 2) && (y0 < f.h - 2), col_ok, D[1]);
+    }
 #pragma unroll
     for (int st = 0; st < TH / 2; ++st) {
-        const int yA = y0 + 2 * st;
-        decode_row(2 * st + 6);
-        decode_row(2 * st + 7);
-        if (MODE != MODE_HQLI) { d_row_at(2 * st + 2); d_row_at(2 * st + 3); }
-        int R[2][4], G[2][4], B[2][4];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int* p = P[2 * st + 3 + a];
-            const int* pu = P[2 * st + 2 + a];
-            const int* pd = P[2 * st + 4 + a];
-            int hl = 0, hr = 0, ul = 0, ur = 0, dl = 0, dr = 0;
-            if (MODE == MODE_HQLI) {
-                hl = from_left(p[3]); hr = from_right(p[0]);
-                ul = from_left(pu[3]); ur = from_right(pu[0]);
-                dl = from_left(pd[3]); dr = from_right(pd[0]);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int c = p[k];
-                int r, g, b;
-                if (MODE == MODE_HQLI) {
-                    const int s = role<PAT>(a, k & 1);
-                    const int le = k > 0 ? p[k - 1] : hl, ri = k < 3 ? p[k + 1] : hr;
-                    const int up = pu[k], dn = pd[k];
-                    if (s != 1) {
-                        const int ul_ = k > 0 ? pu[k - 1] : ul, ur_ = k < 3 ? pu[k + 1] : ur;
-                        const int dl_ = k > 0 ? pd[k - 1] : dl, dr_ = k < 3 ? pd[k + 1] : dr;
-                        const int gg = (le + ri + up + dn + 2) >> 2;
-                        const int oth = (ul_ + ur_ + dl_ + dr_ + 2) >> 2;
-                        if (s == 0) { r = c; g = gg; b = oth; } else { b = c; g = gg; r = oth; }
-                    } else {
-                        const int hz = (le + ri + 1) >> 1, vt = (up + dn + 1) >> 1;
-                        g = c;
-                        if (a == r_row_of(PAT)) { r = hz; b = vt; } else { r = vt; b = hz; }
-                    }
-                    r = imin(r, ch); g = imin(g, ch); b = imin(b, ch);
-                } else {
-                    mg_px<PAT, MODE>(a, k, c, D[2 * st + a], D[2 * st + a + 1], D[2 * st + a + 2], ch, r, g, b);
-                    if (CAREFUL) {
-                        // positions the colour-difference pass never reaches keep R = B = 0
-                        const bool core = col_core[k] && (yA + a >= 1) && (yA + a < h - 1);
-                        r = core ? r : 0;
-                        b = core ? b : 0;
-                    }
-                }
-                R[a][k] = r; G[a][k] = g; B[a][k] = b;
-            }
-        }
-        // w % 4 == 2: column w-3 of the right-edge column fill lives in the
-        // lane to the left (DPP while every lane is active)
-        int LR[2] = {0, 0}, LG[2] = {0, 0}, LB[2] = {0, 0};
-        if (ODD && CAREFUL) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                LR[a] = from_left(R[a][3]);
-                LG[a] = from_left(G[a][3]);
-                LB[a] = from_left(B[a][3]);
-            }
-        }
-
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int y = yA + a;
-            if (!CAREFUL) {
-                // staged rows need 4-byte aligned runs: always for u16, for u8 when w % 4 == 0
-                if (!ODD || f.out16) {
-                    const int lane = threadIdx.x & 63;
-                    store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)(x + 4 - 4 * lane), R[a],
-                                     G[a], B[a], lane >= 1 && lane <= 62, f.out16 ? 1488u : 744u);
-                } else if (lane_out) {
-                    store4<ODD>(f, (uint32_t)x, (uint32_t)y, R[a], G[a], B[a]);
-                }
-                continue;
-            }
-            if (y >= h || y < 2 || y >= h - 2) continue;  // the same for every lane
-            int r[4], g[4], b[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { r[k] = R[a][k]; g[k] = G[a][k]; b[k] = B[a][k]; }
-            if (MODE == MODE_HQLI) {
-                // pre-fill value of a border column: never written -> 0
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (!col_ok[k]) { r[k] = 0; g[k] = 0; b[k] = 0; }
-            }
-            if (lane_out) {
-                if (y == 2) { store4<ODD>(f, x, 0, r, g, b); store4<ODD>(f, x, 1, r, g, b); }
-                if (y == h - 3) { store4<ODD>(f, x, h - 2, r, g, b); store4<ODD>(f, x, h - 1, r, g, b); }
-            }
-            // column fill of the middle rows: columns 0-1 <- 2, w-2..w-1 <- w-3
-            if (x == 0) { r[0] = r[1] = R[a][2]; g[0] = g[1] = G[a][2]; b[0] = b[1] = B[a][2]; }
-            if (!ODD && x == w - 4) { r[2] = r[3] = R[a][1]; g[2] = g[3] = G[a][1]; b[2] = b[3] = B[a][1]; }
-            if (ODD && x == w - 2) { r[0] = r[1] = LR[a]; g[0] = g[1] = LG[a]; b[0] = b[1] = LB[a]; }
-            if (!ODD || f.out16) {
-                // staged like the interior rows, clipped at the frame's right edge
-                const int lane = threadIdx.x & 63;
-                const int xs = x + 4 - 4 * lane;
-                const uint32_t cols = (uint32_t)imin(248, w - xs);
-                store_row_staged(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)xs, r, g, b, lane_out,
-                                 cols * (f.out16 ? 6u : 3u));
-            } else if (lane_out) {
-                store4<ODD>(f, x, y, r, g, b);
-            }
-        }
+        decode(2 * st + 6);
+        decode(2 * st + 7);
+        band_step<FMT, PAT, MODE, CAREFUL, ODD>(f, sp, x, y0 + 2 * st, stage, &P[2 * st + 2], &D[2 * st], col_ok,
+                                                col_core);
     }
 }
 
@@ -570,11 +650,9 @@ __device__ __forceinline__ void run_band(const FastParams& f, int x, int y0, boo
 
 // Wave -> (strip, band).  Blocks are spread round-robin over the 8 XCDs; the
 // remap gives each XCD a contiguous run of bands so that the 6-row aprons
-// shared by vertically adjacent bands are read through one L2.
-template <int FMT, int PAT, int MODE, int TH, bool ODD>
-__global__ __launch_bounds__(256) void k_fast(FastParams f) {
-    __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
-    const int lane = threadIdx.x & 63;
+// shared by vertically adjacent bands are read through one L2.  False for the
+// waves past the last band.
+__device__ __forceinline__ bool wave_place(const FastParams& f, int& sx, int& by) {
     const int wpb = blockDim.x >> 6;
     const int nblk = gridDim.x;
     // logical block: consecutive within an XCD (or the launch order).  With
@@ -584,22 +662,28 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     const int q8 = nblk >> 3, r8 = nblk & 7;
     const int lblk = (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x;
     const int wave = lblk * wpb + (threadIdx.x >> 6);
-    if (lblk >= nblk || wave >= f.nwaves) return;
+    if (lblk >= nblk || wave >= f.nwaves) return false;
     // wave order: band-fastest (consecutive waves walk down a strip) or
     // strip-fastest (consecutive waves cover a row band left to right)
     const bool strip_fastest = (f.xcd_remap & 2) != 0;
-    const int by = strip_fastest ? wave / f.nsx : wave % f.nby;
-    const int sx = strip_fastest ? wave % f.nsx : wave / f.nby;
-    const int x = sx * 248 - 4 + 4 * lane;
+    by = strip_fastest ? wave / f.nsx : wave % f.nby;
+    sx = strip_fastest ? wave % f.nsx : wave / f.nby;
+    return true;
+}
+
+template <int FMT, int PAT, int MODE, int TH, bool ODD>
+__global__ __launch_bounds__(256) void k_fast(FastParams f) {
+    __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
+    int sx, by;
+    if (!wave_place(f, sx, by)) return;
+    const Strip sp = strip_of(f, sx);
     const int y0 = by * TH;
-    const bool lane_out = lane >= 1 && lane <= 62 && x < f.w;
-    const int xs0 = sx * 248, xs1 = xs0 + 248;
     // careful: the band's outputs touch the 3-pixel frame rim, or its raw
     // window reaches the stream's final row
-    const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || xs0 < 3 || xs1 > f.w - 3;
+    const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
     uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, x, y0, lane_out, stage);
-    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, x, y0, lane_out, stage);
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, sp, y0, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, sp, y0, stage);
 }
 
 // ---------------------------------------------------------------- launchers

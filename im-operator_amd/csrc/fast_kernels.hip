@@ -8,19 +8,29 @@
 
 namespace imop_hip {
 
-// Launch shape (set by the autotuner through imop_hip_set_launch_shape).
-static std::atomic<int> g_rows{4}, g_wpb{4}, g_xcd{3};
+// Launch shape (set by the autotuner through imop_hip_set_launch_shape), one
+// word so that a launch never sees half of a concurrent change:
+// rows | wpb << 16 | xcd_remap << 24.  xcd_remap bit 0: XCD-contiguous block
+// order; bit 1: strip-fastest wave order.
+static std::atomic<uint32_t> g_shape{4u | (4u << 16) | (3u << 24)};
 
 void set_launch_shape(int rows, int wpb, int xcd_remap) {
-    if (rows == 2 || rows == 4 || rows == 8 || rows == 16) g_rows = rows;
-    if (wpb == 1 || wpb == 2 || wpb == 4) g_wpb = wpb;
-    if (xcd_remap >= 0 && xcd_remap <= 3) g_xcd = xcd_remap;
+    uint32_t cur = g_shape.load();
+    for (;;) {
+        int r = (int)(cur & 0xFFFFu), w = (int)((cur >> 16) & 0xFFu), x = (int)(cur >> 24);
+        if (rows == 2 || rows == 4 || rows == 8 || rows == 16) r = rows;
+        if (wpb == 1 || wpb == 2 || wpb == 4) w = wpb;
+        if (xcd_remap >= 0 && xcd_remap <= 3) x = xcd_remap;
+        const uint32_t next = (uint32_t)r | ((uint32_t)w << 16) | ((uint32_t)x << 24);
+        if (g_shape.compare_exchange_weak(cur, next)) return;
+    }
 }
 
 void get_launch_shape(int* rows, int* wpb, int* xcd_remap) {
-    if (rows) *rows = g_rows;
-    if (wpb) *wpb = g_wpb;
-    if (xcd_remap) *xcd_remap = g_xcd;
+    const uint32_t s = g_shape.load();
+    if (rows) *rows = (int)(s & 0xFFFFu);
+    if (wpb) *wpb = (int)((s >> 16) & 0xFFu);
+    if (xcd_remap) *xcd_remap = (int)(s >> 24);
 }
 
 hipError_t launch_fast_u8(const FastParams& f, int mode, hipStream_t st);
@@ -42,7 +52,7 @@ bool fast_path_ok(const FrameParams& f) {
     return true;
 }
 
-FastParams make_fast(const FrameParams& f, int th) {
+FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     FastParams q{};
     q.in = f.in;
     q.out = f.out;
@@ -57,16 +67,18 @@ FastParams make_fast(const FrameParams& f, int th) {
     q.bl_on = f.bl_on;
     for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
     q.th = (th == 2 || th == 4 || th == 8) ? th : 16;
-    q.wpb = g_wpb;
-    q.xcd_remap = g_xcd;
-    q.nsx = (f.w + 247) / 248;
+    q.wpb = wpb;
+    q.xcd_remap = xcd;
+    q.nsx = strip_count(f.w, f.out16);
     q.nby = (f.h + q.th - 1) / q.th;
     q.nwaves = q.nsx * q.nby;
     return q;
 }
 
 hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st) {
-    FastParams q = make_fast(f, g_rows);
+    int rows, wpb, xcd;
+    get_launch_shape(&rows, &wpb, &xcd);
+    FastParams q = make_fast(f, rows, wpb, xcd);
     switch (f.fmt) {
     case FMT_U8: return launch_fast_u8(q, mode, st);
     case FMT_U16: return launch_fast_u16(q, mode, st);
