@@ -42,16 +42,19 @@ struct FrameParams {
 };
 
 // Strips of the register-resident kernel.  A wave computes 248 columns
-// [xs, xs + 248), xs = stride * strip.  u8 output: stride 248, each strip
-// writes its own columns.  u16 output: stride 224, and in row y the strip
-// boundaries sit at stride * k + s(y), s(y) in {0, 8, 16, 24} chosen so that
-// each boundary's output byte is 64-byte aligned; every write window then
-// lies inside its wave's computed columns, and no 64-byte sector of the
-// output is written half by one wave and half by another (such sectors
-// stream at ~4.1 TB/s against ~5 TB/s for whole sectors).
-__host__ __device__ inline int strip_stride(int out16) { return out16 ? 224 : 248; }
-__host__ __device__ inline int strip_count(int w, int out16) {
-    const int s = strip_stride(out16);
+// [xs, xs + 248), xs = stride * strip.  Plain strips: stride 248, each strip
+// writes its own columns.  Aligned strips (u16 output of the single-frame
+// kernel): stride 224, and in row y the strip boundaries sit at
+// stride * k + s(y), s(y) in {0, 8, 16, 24} chosen so that each boundary's
+// output byte is 64-byte aligned; every write window then lies inside its
+// wave's computed columns, and no 64-byte sector of the output is written
+// half by one wave and half by another (such sectors stream at ~4.1 TB/s
+// against ~5 TB/s for whole sectors).  The batched kernel keeps plain strips:
+// its per-row scalar arithmetic would not fit the scalar registers of a
+// kernel that inlines every layout.
+__host__ __device__ inline int strip_stride(bool aligned) { return aligned ? 224 : 248; }
+__host__ __device__ inline int strip_count(int w, bool aligned) {
+    const int s = strip_stride(aligned);
     return w <= 248 ? 1 : (w - 248 + s - 1) / s + 1;
 }
 
@@ -65,10 +68,11 @@ struct FastParams {
     int32_t bl_on;
     int32_t bl_i[4];      // integral black level by (y & 1) * 2 + (x & 1)
     int32_t th;           // rows per wave (even)
-    int32_t nsx, nby;     // strips (248 columns each) and bands
+    int32_t nsx, nby;     // strips (strip_count) and bands
     int32_t nwaves;
     int32_t wpb;          // waves per workgroup
     int32_t xcd_remap;    // give each XCD a contiguous run of waves
+    int32_t aligned;      // aligned strips for u16 output (strip_stride)
 };
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         f.bl_on = (bf.flags >> 1) & 1;
         for (int k = 0; k < 4; ++k) f.bl_i[k] = bf.bl_i[k];
         const int nsx = bf.nsx, nby = bf.nby;
+        f.aligned = 0;  // plain strips (strip_stride)
         f.th = TH;
         f.nsx = nsx;
         f.nby = nby;

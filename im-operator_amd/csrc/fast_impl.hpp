@@ -5,8 +5,8 @@
 // band of TH rows.  Lanes 1..62 compute 248 columns; lanes 0 and 63 only
 // load, so that every computing lane finds its horizontal neighbours in the
 // adjacent lanes (DPP wave_shr / wave_shl, one VALU op per value).  Strips
-// advance 248 columns for u8 output; for u16 output they advance 224 and each
-// row's strip boundaries shift so that they fall on 64-byte boundaries of the
+// advance 248 columns, or, for u16 output of the single-frame kernel, 224
+// with each row's strip boundaries shifted onto 64-byte boundaries of the
 // output (strip_stride, row_shift), each wave writing a window of the columns
 // it computed.
 //
@@ -47,8 +47,15 @@ __device__ __forceinline__ int from_right(int v) {  // lane i <- lane i+1
     return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
 }
 
-// clamp(v, 0, hi); with a constant hi the compiler forms one v_med3_i32
-__device__ __forceinline__ int clamp0(int v, int hi) { return imin(imax(v, 0), hi); }
+// clamp(v, 0, hi) in one v_med3_i32.  The compiler forms it by itself only
+// for a constant hi (it cannot prove 0 <= hi for a runtime bound, which is a
+// wave-uniform clamp_hi here, hence the scalar operand).
+__device__ __forceinline__ int clamp0(int v, int hi) {
+    if (__builtin_constant_p(hi)) return imin(imax(v, 0), hi);
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "s"(hi));
+    return r;
+}
 
 __device__ __forceinline__ int tdiv4(int v) { return (v + ((v >> 31) & 3)) >> 2; }           // C's v / 4
 __device__ __forceinline__ int tdiv2(int v) { return (v + (int)((unsigned)v >> 31)) >> 1; }  // C's v / 2
@@ -182,6 +189,24 @@ __device__ __forceinline__ void raw_values(Raw r, bool sub2, uint32_t v[4]) {
     }
 }
 
+// 12-bit packed run (bytes b0..b5 = samples P0..P3, P0 = b0 << 4 | b1 & 15,
+// P1 = b2 << 4 | b1 >> 4, P2 = b3 << 4 | b4 & 15, P3 = b5 << 4 | b4 >> 4) to
+// black-levelled samples, two at a time in 16-bit halves: byte permutes
+// gather (b0, b3), (b1, b4) and (b1 b2, b4 b5); X = (P0, P2), Y = (P1, P3);
+// the saturating 16-bit subtract is max(0, v - offset) for v < 4096.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void p12_samples(Raw r, int o0, int o1, uint32_t v[4]) {
+    const uint32_t a = __builtin_amdgcn_perm(r.hi, r.lo, 0x0C030C00u);  // b0, 0, b3, 0
+    const uint32_t b = __builtin_amdgcn_perm(r.hi, r.lo, 0x0C040C01u);  // b1, 0, b4, 0
+    const uint32_t c = __builtin_amdgcn_perm(r.hi, r.lo, 0x05040201u);  // b1, b2, b4, b5
+    const u16x2 x = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, (a << 4) | (b & 0x000F000Fu)),
+                                                  (u16x2){(uint16_t)o0, (uint16_t)o0});
+    const u16x2 y = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, c) >> (u16x2){4, 4},
+                                                  (u16x2){(uint16_t)o1, (uint16_t)o1});
+    v[0] = x.x; v[1] = y.x; v[2] = x.y; v[3] = y.y;
+}
+
 // Black level (integral offsets o0 / o1 for even / odd columns of the row)
 // or MSB alignment applied to 4 raw values.
 template <int FMT, bool MASK>
@@ -294,16 +319,18 @@ struct Strip {
     bool first, last;
 };
 
+__device__ __forceinline__ bool aligned_strips(const FastParams& f) { return f.out16 && f.aligned; }
+
 __device__ __forceinline__ Strip strip_of(const FastParams& f, int sx) {
-    return Strip{sx * strip_stride(f.out16), sx == 0, sx == f.nsx - 1};
+    return Strip{sx * strip_stride(aligned_strips(f)), sx == 0, sx == f.nsx - 1};
 }
 
-// s(y): u16 output, the shift in {0, 8, 16, 24} that puts the row's strip
+// s(y): aligned strips, the shift in {0, 8, 16, 24} that puts the row's strip
 // boundaries on 64-byte boundaries (48 * s/8 == -(out + 6 * y * w) mod 64
 // needs that address to be a multiple of 16; otherwise 0).  Only the low
 // bits of the address matter, so 32-bit wrap-around is harmless.
 __device__ __forceinline__ int row_shift(const FastParams& f, int y) {
-    if (!f.out16) return 0;
+    if (!aligned_strips(f)) return 0;
     const uint32_t a = (uint32_t)(uintptr_t)f.out + 6u * (uint32_t)y * (uint32_t)f.w;
     if (a & 15u) return 0;
     const uint32_t t = ((64u - (a & 63u)) & 63u) >> 4;  // -a / 16 (mod 4)
@@ -313,7 +340,7 @@ __device__ __forceinline__ int row_shift(const FastParams& f, int y) {
 __device__ __forceinline__ void seg_cols(const FastParams& f, const Strip& sp, int y, int& lo, int& hi) {
     const int sh = row_shift(f, y);
     lo = sp.first ? 0 : sp.xs + sh;
-    hi = sp.last ? f.w : imin(sp.xs + strip_stride(f.out16) + sh, f.w);
+    hi = sp.last ? f.w : imin(sp.xs + strip_stride(aligned_strips(f)) + sh, f.w);
 }
 
 // This lane's 4 pixels at x lie in the row's write window (windows start at
@@ -334,6 +361,13 @@ __device__ __forceinline__ bool lane_writes(const FastParams& f, const Strip& sp
 __device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* stage, int lane, uint32_t row_px,
                                                  int c0, int c1, const int r[4], const int g[4], const int b[4],
                                                  bool wr) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    // window arithmetic is wave-uniform: pinned to scalar registers before
+    // the lane-divergent staging writes
+    const uint32_t e = __builtin_amdgcn_readfirstlane(f.out16 ? 6u : 3u);
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(e * (uint32_t)c0);
+    const uint32_t nbytes = __builtin_amdgcn_readfirstlane(e * (uint32_t)(c1 - c0));
+    uint8_t* base = f.out + (uint64_t)e * (row_px + (uint32_t)c0);
     if (f.out16) {
         if (wr) {
             uint2* p = (uint2*)(stage + 6 * (lane - 1));
@@ -349,18 +383,20 @@ __device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* 
             p[2] = (uint32_t)b[2] | ((uint32_t)r[3] << 8) | ((uint32_t)g[3] << 16) | ((uint32_t)b[3] << 24);
         }
     }
-    const uint32_t e = f.out16 ? 6u : 3u;
-    const uint32_t b0 = e * (uint32_t)c0, nbytes = e * (uint32_t)(c1 - c0);
-    const uint32_t i0 = b0 >> 4;
-    const bool two = lane < 32 && 16u * (64u + (uint32_t)lane) < nbytes;
+    const uint32_t o0 = 16u * (uint32_t)lane, o1 = 1024u + o0;
+    const bool two = lane < 32 && o1 < nbytes;
     lds_order();
-    const uint4 v0 = ((const uint4*)stage)[i0 + (uint32_t)lane];
+    const uint4 v0 = ((const uint4*)stage)[(b0 >> 4) + (uint32_t)lane];
     uint4 v1 = make_uint4(0u, 0u, 0u, 0u);
-    if (two) v1 = ((const uint4*)stage)[i0 + 64u + (uint32_t)lane];
+    if (two) v1 = ((const uint4*)stage)[(b0 >> 4) + 64u + (uint32_t)lane];
     lds_order();
-    uint8_t* base = f.out + e * (row_px + (uint32_t)c0);
-    store_chunk(base, 16u * (uint32_t)lane, v0, nbytes);
-    if (two) store_chunk(base, 1024u + 16u * (uint32_t)lane, v1, nbytes);
+    if ((nbytes & 15u) == 0u) {  // whole chunks (every interior u16 window)
+        if (o0 < nbytes) __builtin_nontemporal_store((u32x4){v0.x, v0.y, v0.z, v0.w}, (u32x4*)(base + o0));
+        if (two) __builtin_nontemporal_store((u32x4){v1.x, v1.y, v1.z, v1.w}, (u32x4*)(base + o1));
+    } else {
+        store_chunk(base, o0, v0, nbytes);
+        if (two) store_chunk(base, o1, v1, nbytes);
+    }
 }
 
 // MG colour difference of the two R/B sites a lane owns in one row.
@@ -475,6 +511,12 @@ __device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Ra
     if (ODD && CAREFUL && y == f.h - 1) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < f.w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
+    } else if (FMT == FMT_P12) {
+        p12_samples(raw, o0, o1, v);  // black level included
+        const bool ok = !CAREFUL || (row_ok && lane_ok);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[k] = ok ? (int)v[k] : 0;
+        return;
     } else {
         raw_values<FMT>(raw, ODD && YODD, v);  // w % 4 == 2: odd rows start at sub-pixel 2
     }
@@ -661,7 +703,8 @@ __device__ __forceinline__ bool wave_place(const FastParams& f, int& sx, int& by
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int q8 = nblk >> 3, r8 = nblk & 7;
     const int lblk = (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x;
-    const int wave = lblk * wpb + (threadIdx.x >> 6);
+    // wave-uniform: keeps the band / strip and every row's address arithmetic in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(lblk * wpb + (int)(threadIdx.x >> 6));
     if (lblk >= nblk || wave >= f.nwaves) return false;
     // wave order: band-fastest (consecutive waves walk down a strip) or
     // strip-fastest (consecutive waves cover a row band left to right)

@@ -69,7 +69,8 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     q.th = (th == 2 || th == 4 || th == 8) ? th : 16;
     q.wpb = wpb;
     q.xcd_remap = xcd;
-    q.nsx = strip_count(f.w, f.out16);
+    q.aligned = 1;
+    q.nsx = strip_count(f.w, f.out16 != 0);
     q.nby = (f.h + q.th - 1) / q.th;
     q.nwaves = q.nsx * q.nby;
     return q;

@@ -117,7 +117,7 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         FrameParams f = make_params(d.in, d.in_bytes, d.out, d.width, d.height, d.pattern, d.bit_depth, d.is_packed,
                                     d.black_level, d.black_level_mode, d.algorithm, &mode);
         f.out16 = out_elem_of(d) == 2;
-        const int nsx = strip_count(d.width, f.out16), nby = (d.height + P.rows - 1) / P.rows;
+        const int nsx = strip_count(d.width, false), nby = (d.height + P.rows - 1) / P.rows;
         // k_batch runs MG 8-bit frames on the integer path; a packed 8-bit
         // frame without black level takes the exact path, so it runs alone
         const bool mode_ok = d.algorithm == kHqli || (d.algorithm == kMg && (f.fmt == FMT_U8) == (mode == MODE_MG_INT8));
