@@ -2,10 +2,19 @@
 # Build the reference's own CPU demosaic / black-level path from its sources
 # where they lie, into oracle/_ref/libimop_ref.so.  Test infrastructure only.
 #
-# Flags follow the reference build (CMakeLists.txt:30-46, 222-240): ISO C++17
-# (no GNU extensions, hence no FP contraction), -O3, -mavx2 -mfma, OpenMP.
+# Flags follow the reference build (CMakeLists.txt:30-46, 222-240): C++17,
+# -O3, -mavx2 -mfma, OpenMP.  GCC's default for C++ is -ffp-contract=fast, so
+# multiply-adds are fused into FMAs exactly as in the reference build.
 # -march=native and -flto are left out so the library also runs on other x86
 # hosts; neither changes results without -ffast-math.
+#
+# A second library, libimop_ref_isp.so, holds the CPU white balance, colour
+# matrix and gamma stages.  Their module dispatch files (needed for the input
+# validation they call) also reference the CUDA entry points, which a build
+# without CUDA does not define; compiling with hidden visibility and one
+# section per function and linking with --gc-sections drops the unused
+# dispatchers and with them those references, so only the stages the shim
+# calls remain.
 #
 # Nothing is copied out of the reference tree: the compiler reads the sources
 # in place and only the shared object is written, under oracle/_ref/.
@@ -40,3 +49,18 @@ g++ -std=c++17 -O3 -mavx2 -mfma -fopenmp -fPIC -shared -w \
     "${SRCS[@]}" -o "$OUT/libimop_ref.so.tmp"
 mv "$OUT/libimop_ref.so.tmp" "$OUT/libimop_ref.so"
 echo "build_ref: wrote $OUT/libimop_ref.so"
+
+ISP_SRCS=(
+    "$REF/src/white_balance/manual.cpp"
+    "$REF/src/white_balance/dispatch.cpp"
+    "$REF/src/ccm/linear_3x3.cpp"
+    "$REF/src/ccm/dispatch.cpp"
+    "$REF/src/tone/gamma.cpp"
+    "$REF/src/tone/dispatch.cpp"
+    "$HERE/ref_isp_shim.cpp"
+)
+g++ -std=c++17 -O3 -mavx2 -mfma -fPIC -shared -w -fvisibility=hidden -ffunction-sections -fdata-sections \
+    -Wl,--gc-sections -I"$REF/include" -I"$REF/src" \
+    "${ISP_SRCS[@]}" -o "$OUT/libimop_ref_isp.so.tmp"
+mv "$OUT/libimop_ref_isp.so.tmp" "$OUT/libimop_ref_isp.so"
+echo "build_ref: wrote $OUT/libimop_ref_isp.so"

@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libimop_ref.so")
+REF_ISP_PATH = os.path.join(HERE, "_ref", "libimop_ref_isp.so")
 
 HQLI, MG, PRISM = 1, 2, 8
 RGGB, BGGR, GRBG, GBRG = 0, 1, 2, 3
@@ -41,6 +42,7 @@ def _ptr(a: np.ndarray):
 
 _oracle = None
 _ref = None
+_ref_isp = None
 
 
 def oracle() -> ctypes.CDLL:
@@ -69,6 +71,14 @@ def oracle() -> ctypes.CDLL:
                                                                       _fp, _fp, _dp, _dp, _dp]
         lib.orc_bilinear_apply64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
         lib.orc_bilinear_apply_t64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
+        lib.isp_wb_manual.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
+        lib.isp_ccm_linear.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
+        lib.isp_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
+        lib.isp_epilogue.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, _fp, ctypes.c_float]
+        for fn in ("isp_wb_manual", "isp_ccm_linear", "isp_gamma", "isp_epilogue"):
+            getattr(lib, fn).restype = None
+        lib.isp_gamma_clamp.argtypes = [ctypes.c_float]
+        lib.isp_gamma_clamp.restype = ctypes.c_float
         _oracle = lib
     return _oracle
 
@@ -91,6 +101,72 @@ def reference() -> ctypes.CDLL:
         lib.ref_unpack.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_u8p]
         _ref = lib
     return _ref
+
+
+def reference_isp_available() -> bool:
+    return os.path.exists(REF_ISP_PATH)
+
+
+def reference_isp() -> ctypes.CDLL:
+    """The reference's compiled white balance / colour matrix / gamma stages."""
+    global _ref_isp
+    if _ref_isp is None:
+        lib = ctypes.CDLL(REF_ISP_PATH)
+        lib.ref_wb_manual.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
+        lib.ref_ccm_linear.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
+        lib.ref_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
+        _ref_isp = lib
+    return _ref_isp
+
+
+def _rgb_stage(lib, fn: str, rgb: np.ndarray, bd: int, arg):
+    """Run one RGB stage of `lib` on an [h, w, 3] u8 / u16 image; returns a new image."""
+    h, w, _ = rgb.shape
+    src = np.ascontiguousarray(rgb)
+    dst = np.zeros_like(src)
+    a = np.ascontiguousarray(arg, dtype=np.float32).ctypes.data_as(_fp) if not isinstance(arg, float) else ctypes.c_float(arg)
+    e = getattr(lib, fn)(src.view(np.uint8).ctypes.data_as(_u8p), dst.view(np.uint8).ctypes.data_as(_u8p), w, h, bd, a)
+    if e not in (None, 0):
+        raise RuntimeError(f"{fn} returned {e}")
+    return dst
+
+
+def isp_wb(rgb, bd, gains):
+    """white_balance::process_manual_wb (src/white_balance/manual.cpp:8-41), restated."""
+    return _rgb_stage(oracle(), "isp_wb_manual", rgb, bd, gains)
+
+
+def isp_ccm(rgb, bd, m):
+    """ccm::process_linear_3x3 (src/ccm/linear_3x3.cpp:9-46), restated."""
+    return _rgb_stage(oracle(), "isp_ccm_linear", rgb, bd, m)
+
+
+def isp_gamma(rgb, bd, gamma):
+    """tone::process_gamma (src/tone/gamma.cpp:10-34), restated."""
+    return _rgb_stage(oracle(), "isp_gamma", rgb, bd, float(gamma))
+
+
+def isp_epilogue(rgb, bd, flags, gains=(1.0, 1.0, 1.0), m=(1, 0, 0, 0, 1, 0, 0, 0, 1), gamma=1.0):
+    """White balance, colour matrix, gamma in that order (flags 1 / 2 / 4), restated."""
+    out = np.ascontiguousarray(rgb).copy()
+    h, w, _ = out.shape
+    oracle().isp_epilogue(out.view(np.uint8).ctypes.data_as(_u8p), w, h, bd, flags,
+                          np.asarray(gains, dtype=np.float32).ctypes.data_as(_fp),
+                          np.asarray(m, dtype=np.float32).ctypes.data_as(_fp),
+                          ctypes.c_float(gamma))
+    return out
+
+
+def ref_wb(rgb, bd, gains):
+    return _rgb_stage(reference_isp(), "ref_wb_manual", rgb, bd, gains)
+
+
+def ref_ccm(rgb, bd, m):
+    return _rgb_stage(reference_isp(), "ref_ccm_linear", rgb, bd, m)
+
+
+def ref_gamma(rgb, bd, gamma):
+    return _rgb_stage(reference_isp(), "ref_gamma", rgb, bd, float(gamma))
 
 
 def rgb_bytes(w: int, h: int, bd: int) -> int:

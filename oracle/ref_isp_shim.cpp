@@ -1,0 +1,36 @@
+// ref_isp_shim.cpp -- C entry points over the reference's own CPU white
+// balance, colour matrix and gamma stages, so tests can call the compiled
+// reference through ctypes.  TEST INFRASTRUCTURE ONLY: built by
+// oracle/build_ref.sh together with the reference sources under
+// /root/reference into oracle/_ref/libimop_ref_isp.so, used to pin
+// isp_oracle.c.
+#include <cstdint>
+
+#include "ccm/algorithms.hpp"
+#include "tone/algorithms.hpp"
+#include "white_balance/algorithms.hpp"
+
+#define REF_EXPORT extern "C" __attribute__((visibility("default")))
+
+// white_balance::process_manual_wb, src/white_balance/manual.cpp:8-41
+REF_EXPORT int ref_wb_manual(const uint8_t* in, uint8_t* out, int w, int h, int bd, const float* gains) {
+    white_balance::WBCoefficients c;
+    c.r_gain = gains[0];
+    c.g_gain = gains[1];
+    c.b_gain = gains[2];
+    return static_cast<int>(white_balance::process_manual_wb(in, out, w, h, 3, bd, 0.0f, c));
+}
+
+// ccm::process_linear_3x3, src/ccm/linear_3x3.cpp:9-46
+REF_EXPORT int ref_ccm_linear(const uint8_t* in, uint8_t* out, int w, int h, int bd, const float* m) {
+    ccm::CCMatrix3x3 mat;
+    for (int k = 0; k < 9; ++k) mat.m[k] = m[k];
+    return static_cast<int>(ccm::process_linear_3x3(in, out, w, h, 3, bd, &mat));
+}
+
+// tone::process_gamma, src/tone/gamma.cpp:10-34
+REF_EXPORT int ref_gamma(const uint8_t* in, uint8_t* out, int w, int h, int bd, float gamma) {
+    tone::ToneParams p;
+    p.gamma = gamma;
+    return static_cast<int>(tone::process_gamma(in, out, w, h, 3, bd, p));
+}

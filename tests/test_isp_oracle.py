@@ -1,0 +1,108 @@
+"""The RGB epilogue oracle (oracle/isp_oracle.c: manual white balance, linear
+3x3 colour matrix, gamma) against the compiled reference stages
+(src/white_balance/manual.cpp, src/ccm/linear_3x3.cpp, src/tone/gamma.cpp,
+built by oracle/build_ref.sh) and against closed forms."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+BDS = [8, 10, 12, 14, 16]
+GAINS = [[1.0, 1.0, 1.0], [1.9, 0.7, 2.3], [0.51, 1.0, 3.99], [-1.0, 3e9, float("nan")], [1e-30, 0.0, 65535.0]]
+MATRICES = [[1.6, -0.4, -0.2, -0.3, 1.5, -0.2, 0.05, -0.55, 1.5],
+            [0.299, 0.587, 0.114, -0.169, -0.331, 0.5, 0.5, -0.419, -0.081],
+            [2.5, 1e-3, -7.25, 0.0, 1.0, 0.0, 1e6, -1e6, 0.5]]
+GAMMAS = [2.2, 0.45, 1.0, 1.8, 0.05, 12.0, float("nan")]
+
+needs_ref = pytest.mark.skipif(not O.reference_isp_available(), reason="oracle/_ref not built (reference absent)")
+
+
+def _img(bd, seed, h=41, w=67):
+    rng = np.random.default_rng(seed)
+    mx = (1 << bd) - 1
+    img = rng.integers(0, mx + 1, size=(h, w, 3))
+    img[0, :4] = [[0, 0, 0], [mx, mx, mx], [0, mx, 0], [1, mx - 1, 2]]  # the range ends
+    return img.astype(np.uint8 if bd <= 8 else np.uint16)
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", BDS)
+@pytest.mark.parametrize("gi", range(len(GAINS)))
+def test_white_balance_matches_reference(bd, gi):
+    img = _img(bd, 10 * bd + gi)
+    np.testing.assert_array_equal(O.isp_wb(img, bd, GAINS[gi]), O.ref_wb(img, bd, GAINS[gi]))
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", BDS)
+@pytest.mark.parametrize("mi", range(len(MATRICES) + 2))
+def test_colour_matrix_matches_reference(bd, mi):
+    m = MATRICES[mi] if mi < len(MATRICES) else list(np.random.default_rng(mi).normal(0, 2, 9))
+    img = _img(bd, 20 * bd + mi)
+    np.testing.assert_array_equal(O.isp_ccm(img, bd, m), O.ref_ccm(img, bd, m))
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", BDS)
+@pytest.mark.parametrize("gamma", GAMMAS)
+def test_gamma_matches_reference(bd, gamma):
+    img = _img(bd, 30 * bd)
+    np.testing.assert_array_equal(O.isp_gamma(img, bd, gamma), O.ref_gamma(img, bd, gamma))
+
+
+@needs_ref
+def test_gamma_every_input_value_16_bit():
+    """Every 16-bit input value once (the stage is a function of the value)."""
+    v = np.arange(65536, dtype=np.uint16)
+    img = np.stack([v, v[::-1], v], axis=-1).reshape(256, 256, 3)
+    for gamma in (2.2, 0.45, 1.0 / 2.4):
+        np.testing.assert_array_equal(O.isp_gamma(img, 16, gamma), O.ref_gamma(img, 16, gamma))
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", [8, 12])
+def test_epilogue_is_the_three_stages_in_order(bd):
+    img = _img(bd, 7)
+    g, m, gm = GAINS[1], MATRICES[0], 2.2
+    want = O.ref_gamma(O.ref_ccm(O.ref_wb(img, bd, g), bd, m), bd, gm)
+    np.testing.assert_array_equal(O.isp_epilogue(img, bd, 7, g, m, gm), want)
+    np.testing.assert_array_equal(O.isp_epilogue(img, bd, 5, g, m, gm), O.ref_gamma(O.ref_wb(img, bd, g), bd, gm))
+    np.testing.assert_array_equal(O.isp_epilogue(img, bd, 2, g, m, gm), O.ref_ccm(img, bd, m))
+
+
+@pytest.mark.parametrize("bd", BDS)
+def test_closed_forms(bd):
+    img = _img(bd, 3)
+    mx = (1 << bd) - 1
+    np.testing.assert_array_equal(O.isp_wb(img, bd, [1.0, 1.0, 1.0]), img)
+    np.testing.assert_array_equal(O.isp_ccm(img, bd, [1, 0, 0, 0, 1, 0, 0, 0, 1]), img)
+    np.testing.assert_array_equal(O.isp_gamma(img, bd, 1.0), img)
+    # a permutation matrix permutes the channels exactly
+    np.testing.assert_array_equal(O.isp_ccm(img, bd, [0, 0, 1, 1, 0, 0, 0, 1, 0]), img[..., [2, 0, 1]])
+    # integral gains: min(k v, max); negative gains clamp to 0
+    np.testing.assert_array_equal(O.isp_wb(img, bd, [2.0, 3.0, -1.0])[..., :2],
+                                  np.minimum(img[..., :2].astype(np.int64) * [2, 3], mx))
+    assert not O.isp_wb(img, bd, [2.0, 3.0, -1.0])[..., 2].any()
+    # gamma maps 0 -> 0 and max -> max
+    out = O.isp_gamma(img, bd, 2.2)
+    assert (out[img == 0] == 0).all() and (out[img == mx] == mx).all()
+
+
+@needs_ref
+def test_planted_mistake_is_detected():
+    """The unfused float32 form of the colour matrix (each product rounded)
+    differs from the reference somewhere, so the fused order the oracle
+    follows is pinned, not assumed."""
+    bd = 16
+    img = _img(bd, 99, h=128, w=128)
+    m = np.asarray(list(np.random.default_rng(5).normal(0, 2, 9)), dtype=np.float32)
+    f = img.astype(np.float32)
+    naive = np.empty_like(f)
+    for c in range(3):
+        naive[..., c] = (m[3 * c] * f[..., 0] + m[3 * c + 1] * f[..., 1]) + m[3 * c + 2] * f[..., 2]
+    naive = np.clip(np.trunc(naive + np.float32(0.5)), 0, 65535).astype(np.uint16)
+    ref = O.ref_ccm(img, bd, m)
+    assert not np.array_equal(naive, ref)
+    np.testing.assert_array_equal(O.isp_ccm(img, bd, m), ref)
