@@ -58,6 +58,18 @@ __host__ __device__ inline int strip_count(int w, bool aligned) {
     return w <= 248 ? 1 : (w - 248 + s - 1) / s + 1;
 }
 
+// RGB stages that may follow the demosaic in the same launch (epilogue.hpp).
+enum EpilogueFlag : int { EPI_WB = 1, EPI_CCM = 2, EPI_GAMMA = 4 };
+
+struct Epilogue {
+    int32_t flags;         // EpilogueFlag bits; 0 = none
+    int32_t maxv;          // detail::safe_max_val(bit depth)
+    float gain[3];         // white balance gains r, g, b
+    float m[9];            // colour matrix, row-major, out = M * in
+    uint32_t lut_max;      // last gamma table entry: 255 (u8 RGB) or 65535 (u16 RGB)
+    const uint16_t* lut;   // gamma table for every storable value 0 .. lut_max (device)
+};
+
 // Parameters of the register-resident HQLI / MG kernel (fast_kernels.hip).
 struct FastParams {
     const uint8_t* in;
@@ -73,6 +85,7 @@ struct FastParams {
     int32_t wpb;          // waves per workgroup
     int32_t xcd_remap;    // give each XCD a contiguous run of waves
     int32_t aligned;      // aligned strips for u16 output (strip_stride)
+    Epilogue epi;         // RGB stages applied before the stores (k_fast<.., EPI = true>)
 };
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }

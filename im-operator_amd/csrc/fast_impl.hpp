@@ -33,6 +33,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "epilogue.hpp"
 #include "plain.hpp"
 
 namespace imop_hip {
@@ -536,7 +537,7 @@ __device__ __forceinline__ void lane_columns(int x, int w, bool col_ok[4], bool 
 // One step: output rows yA, yA + 1 (yA even) of the lane run at x.  Pw holds
 // the decoded rows yA-1 .. yA+4; Dw the MG colour differences of rows yA-1,
 // yA on entry, and the step forms rows yA+1, yA+2 into Dw[2], Dw[3].
-template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD>
+template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD, bool EPI>
 __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, int x, int yA, uint32_t* stage,
                                           const int (*Pw)[4], int (*Dw)[3], const bool col_ok[4],
                                           const bool col_core[4]) {
@@ -589,6 +590,14 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
             }
             R[a][k] = r; G[a][k] = g; B[a][k] = b;
         }
+    }
+    // RGB epilogue (white balance, colour matrix, gamma) on every pixel
+    // computed, before the border fill copies any of them
+    if (EPI) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) epi_px(f.epi, R[a][k], G[a][k], B[a][k]);
     }
     // w % 4 == 2: column w-3 of the right-edge column fill lives in the
     // lane to the left (DPP while every lane is active)
@@ -654,7 +663,7 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
 
 // One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
 // Every load of the band is issued up front.
-template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD>
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false>
 __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
     const int x = sp.xs - 4 + 4 * (int)(threadIdx.x & 63);
@@ -683,8 +692,8 @@ __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, i
     for (int st = 0; st < TH / 2; ++st) {
         decode(2 * st + 6);
         decode(2 * st + 7);
-        band_step<FMT, PAT, MODE, CAREFUL, ODD>(f, sp, x, y0 + 2 * st, stage, &P[2 * st + 2], &D[2 * st], col_ok,
-                                                col_core);
+        band_step<FMT, PAT, MODE, CAREFUL, ODD, EPI>(f, sp, x, y0 + 2 * st, stage, &P[2 * st + 2], &D[2 * st],
+                                                     col_ok, col_core);
     }
 }
 
@@ -714,7 +723,7 @@ __device__ __forceinline__ bool wave_place(const FastParams& f, int& sx, int& by
     return true;
 }
 
-template <int FMT, int PAT, int MODE, int TH, bool ODD>
+template <int FMT, int PAT, int MODE, int TH, bool ODD, bool EPI = false>
 __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
     int sx, by;
@@ -725,8 +734,8 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     // window reaches the stream's final row
     const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
     uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD>(f, sp, y0, stage);
-    else run_band<FMT, PAT, MODE, false, TH, ODD>(f, sp, y0, stage);
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD, EPI>(f, sp, y0, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD, EPI>(f, sp, y0, stage);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -734,6 +743,10 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
 template <int FMT, int PAT, int MODE, bool ODD>
 static void launch_fmpo(const FastParams& f, hipStream_t st) {
     const int blocks = (f.nwaves + f.wpb - 1) / f.wpb;
+    if (f.epi.flags) {  // the epilogue form is built for bands of 4 rows only
+        hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, ODD, true>), dim3(blocks), dim3(64 * f.wpb), 0, st, f);
+        return;
+    }
     switch (f.th) {
     case 2: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
     case 4: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;

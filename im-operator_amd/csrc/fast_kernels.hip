@@ -76,10 +76,12 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     return q;
 }
 
-hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st) {
+hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epilogue* epi) {
     int rows, wpb, xcd;
     get_launch_shape(&rows, &wpb, &xcd);
-    FastParams q = make_fast(f, rows, wpb, xcd);
+    const bool with_epi = epi && epi->flags;
+    FastParams q = make_fast(f, with_epi ? 4 : rows, wpb, xcd);
+    if (with_epi) q.epi = *epi;
     switch (f.fmt) {
     case FMT_U8: return launch_fast_u8(q, mode, st);
     case FMT_U16: return launch_fast_u16(q, mode, st);

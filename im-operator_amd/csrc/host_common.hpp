@@ -35,7 +35,16 @@ hipError_t get_scratch(size_t need, void** out);
 
 // Enqueue one frame: PRISM (workspace `prism_ws`, or the per-device one when
 // null), the register-resident kernel when it applies, else the LDS-tiled one.
-int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st);
+// epi (optional): the RGB epilogue, fused into the register-resident kernel
+// or run as one in-place pass after the demosaic.
+int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st,
+                 const Epilogue* epi = nullptr);
+
+// Checks, parameters and launch of one frame from device buffers
+// (imop_hip_demosaic; with an epilogue, imop_hip_demosaic_isp).
+int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int w, int h, int pattern,
+                 int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st,
+                 const Epilogue* epi = nullptr);
 
 // Number of visible devices (probed once).
 int device_count_cached();

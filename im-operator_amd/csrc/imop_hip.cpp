@@ -7,6 +7,7 @@
 #include <mutex>
 #include <vector>
 
+#include "epilogue.hpp"
 #include "host_common.hpp"
 #include "imop_hip.h"
 #include "plain.hpp"
@@ -137,8 +138,13 @@ hipError_t get_scratch(size_t need, void** out) {
     return hipSuccess;
 }
 
-int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st) {
+int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st, const Epilogue* epi) {
     hipError_t he;
+    const bool with_epi = epi && epi->flags;
+    if (with_epi && algo != kPrism && fast_path_ok(f)) {
+        he = launch_fast(f, mode, st, epi);
+        return he == hipSuccess ? kOk : kInternal;
+    }
     if (algo == kPrism) {
         // the PRISM passes index their planes with 32-bit offsets
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;
@@ -153,11 +159,12 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
     } else {
         he = launch_plain(f, mode, st);
     }
+    if (he == hipSuccess && with_epi) he = launch_rgb_epilogue(f.out, (uint64_t)f.w * (uint64_t)f.h, f.out16, *epi, st);
     return he == hipSuccess ? kOk : kInternal;
 }
 
 int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int w, int h, int pattern,
-                 int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st) {
+                 int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st, const Epilogue* epi) {
     int e = check_args(d_in, d_out, w, h, pattern, algo, bd, packed, bl, bl_mode);
     if (e != kOk) return e;
     if (in_bytes < bayer_bytes(w, h, bd, packed != 0) || out_bytes < rgb_bytes(w, h, bd)) return kInternal;
@@ -165,7 +172,7 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
     if (bd > 8 && ((!packed && ((uintptr_t)d_in & 1)) || ((uintptr_t)d_out & 1))) return kInternal;
     int mode = -1;
     FrameParams f = make_params(d_in, in_bytes, d_out, w, h, pattern, bd, packed, bl, bl_mode, algo, &mode);
-    return launch_frame(f, algo, mode, nullptr, st);
+    return launch_frame(f, algo, mode, nullptr, st, epi);
 }
 
 namespace {

@@ -88,6 +88,32 @@ class FrameDesc(ctypes.Structure):
     ]
 
 
+class RgbEpilogue(ctypes.Structure):
+    """imop_hip_rgb_epilogue (include/imop_hip.h): flags 1 white balance,
+    2 colour matrix, 4 gamma, applied in that order."""
+    _fields_ = [
+        ("flags", ctypes.c_int32),
+        ("wb_gains", ctypes.c_float * 3),
+        ("ccm", ctypes.c_float * 9),
+        ("gamma", ctypes.c_float),
+    ]
+
+
+def rgb_epilogue(gains=None, ccm=None, gamma=None) -> RgbEpilogue:
+    """Epilogue parameters; a stage is on when its argument is given."""
+    e = RgbEpilogue()
+    if gains is not None:
+        e.flags |= 1
+        e.wb_gains[:] = [float(v) for v in gains]
+    if ccm is not None:
+        e.flags |= 2
+        e.ccm[:] = [float(v) for v in np.asarray(ccm, dtype=np.float64).ravel()]
+    if gamma is not None:
+        e.flags |= 4
+        e.gamma = float(gamma)
+    return e
+
+
 _lib = None
 
 
@@ -123,6 +149,10 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_batch_plan_launch.restype = i
         L.imop_hip_batch_plan_destroy.argtypes = [vp]
         L.imop_hip_batch_plan_destroy.restype = i
+        L.imop_hip_demosaic_isp.argtypes = [vp, u64, vp, u64, i, i, i, i, i, i, fp, i, ctypes.POINTER(RgbEpilogue), vp]
+        L.imop_hip_demosaic_isp.restype = i
+        L.imop_hip_rgb_epilogue_apply.argtypes = [vp, u64, i, i, i, ctypes.POINTER(RgbEpilogue), vp]
+        L.imop_hip_rgb_epilogue_apply.restype = i
         L.imop_hip_set_batch_shape.argtypes = [i, i, i, i]
         L.imop_hip_set_batch_shape.restype = i
         L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
@@ -185,6 +215,34 @@ def demosaic(d_in, d_out, width: int, height: int, pattern: int, algorithm: int,
     return lib().imop_hip_demosaic(ctypes.c_void_p(d_in.data_ptr()), in_bytes, ctypes.c_void_p(d_out.data_ptr()),
                                    out_bytes, width, height, int(pattern), int(algorithm), bit_depth,
                                    int(bool(is_packed)), blp, mode, ctypes.c_void_p(stream))
+
+
+def _stream_arg(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+
+def demosaic_isp(d_in, d_out, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,
+                 is_packed: bool = False, black_level=None, per_channel: bool = False,
+                 epilogue: RgbEpilogue | None = None, stream=None) -> int:
+    """imop_hip_demosaic_isp: the demosaic followed by the RGB epilogue
+    (white balance, colour matrix, gamma) on device tensors."""
+    bl, mode = _bl_args(black_level, per_channel)
+    blp = bl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if bl is not None else None
+    ep = ctypes.byref(epilogue) if epilogue is not None else None
+    return lib().imop_hip_demosaic_isp(ctypes.c_void_p(d_in.data_ptr()), d_in.numel() * d_in.element_size(),
+                                       ctypes.c_void_p(d_out.data_ptr()), d_out.numel() * d_out.element_size(),
+                                       width, height, int(pattern), int(algorithm), bit_depth,
+                                       int(bool(is_packed)), blp, mode, ep, ctypes.c_void_p(_stream_arg(stream)))
+
+
+def rgb_epilogue_apply(d_rgb, width: int, height: int, bit_depth: int, epilogue: RgbEpilogue, stream=None) -> int:
+    """imop_hip_rgb_epilogue_apply: the epilogue alone, in place, on a device RGB tensor."""
+    return lib().imop_hip_rgb_epilogue_apply(ctypes.c_void_p(d_rgb.data_ptr()), d_rgb.numel() * d_rgb.element_size(),
+                                             width, height, bit_depth, ctypes.byref(epilogue),
+                                             ctypes.c_void_p(_stream_arg(stream)))
 
 
 def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,

@@ -189,6 +189,47 @@ int imop_hip_isp_chain_bwd(const float* d_raw, int n, int width, int height, int
 int imop_hip_bilinear_fwd(const float* d_x, int width, int height, int pattern, float* d_out, void* stream);
 int imop_hip_bilinear_vjp(const float* d_v, int width, int height, int pattern, float* d_out, void* stream);
 
+/* ------------------------------------------------------ RGB epilogue */
+
+/* The integer-domain RGB stages that follow the demosaic, applied in this
+ * order with the reference CPU semantics (each stage rounds and clamps to
+ * [0, max] of the bit depth, as its separate pass does):
+ *   flag 1: manual white balance, white_balance::process_manual_wb
+ *           (src/white_balance/manual.cpp:8-41), gains r, g, b;
+ *   flag 2: linear 3x3 colour matrix, ccm::process_linear_3x3
+ *           (src/ccm/linear_3x3.cpp:9-46), row-major, out = M * in;
+ *   flag 4: gamma, tone::process_gamma (src/tone/gamma.cpp:10-34), the
+ *           exponent clamped to [0.1, 10] like the reference.
+ * Outputs equal those three CPU functions run one after the other. */
+typedef struct imop_hip_rgb_epilogue {
+    int32_t flags;
+    float wb_gains[3];
+    float ccm[9];
+    float gamma;
+} imop_hip_rgb_epilogue;
+
+/* imop_hip_demosaic followed by the epilogue, in one launch on the
+ * register-resident path (HQLI / MG of even widths) and as one extra
+ * in-place pass otherwise.  Replaces the reference's demosaic_cuda
+ * (src/algorithms/cuda_dispatch.cpp:68-141) followed by
+ * process_white_balance_cuda / process_ccm_cuda / process_tone_cuda
+ * (src/gpu_dispatch.cpp:292-356), each a host round trip of the RGB image.
+ * epilogue NULL or flags 0: imop_hip_demosaic.  The first call with a given
+ * (bit depth, gamma) builds that gamma table on the device with a blocking
+ * copy, so it must not be made during stream capture; up to 64 tables are
+ * kept per process. */
+int imop_hip_demosaic_isp(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
+                          int height, int pattern, int algorithm, int bit_depth, int is_packed,
+                          const float* black_level, int black_level_mode, const imop_hip_rgb_epilogue* epilogue,
+                          void* stream);
+
+/* The epilogue alone, in place, on an interleaved RGB image of width x
+ * height (u8 if bit_depth <= 8, else u16): the reference's
+ * process_white_balance_cuda / process_ccm_cuda / process_tone_cuda
+ * (src/gpu_dispatch.cpp:292-356) with CPU semantics, device buffer. */
+int imop_hip_rgb_epilogue_apply(void* d_rgb, uint64_t rgb_bytes, int width, int height, int bit_depth,
+                                const imop_hip_rgb_epilogue* epilogue, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,23 +74,28 @@ def test_no_device_behaves_like_reference_stub():
     assert imop_amd.demosaic_batch([], stream=0) == (6, -1)
     plan = imop_amd.BatchPlan([])
     assert plan.error == 6
+    ep = imop_amd.rgb_epilogue(gains=[2.0, 1.0, 1.5], gamma=2.2)
+    assert lib.imop_hip_demosaic_isp(None, 0, None, 0, 8, 8, 0, 2, 8, 0, None, 0, ctypes.byref(ep), None) == 6
+    assert lib.imop_hip_rgb_epilogue_apply(None, 0, 8, 8, 8, ctypes.byref(ep), None) == 6
 
 
-def test_frame_desc_layout_matches_header(tmp_path):
-    """The ctypes mirror of imop_hip_frame_desc has the C compiler's layout."""
+@pytest.mark.parametrize("ctype,cname", [(imop_amd.FrameDesc, "imop_hip_frame_desc"),
+                                         (imop_amd.RgbEpilogue, "imop_hip_rgb_epilogue")])
+def test_struct_layout_matches_header(tmp_path, ctype, cname):
+    """The ctypes mirrors of the header's structs have the C compiler's layout."""
     gcc = shutil.which("gcc")
     if gcc is None:
         pytest.skip("gcc not available")
     src = tmp_path / "layout.c"
-    fields = [f[0] for f in imop_amd.FrameDesc._fields_]
+    fields = [f[0] for f in ctype._fields_]
     cnames = ["in" if f == "in_" else f for f in fields]
-    body = "".join(f'printf("%zu\\n", offsetof(imop_hip_frame_desc, {c}));' for c in cnames)
+    body = "".join(f'printf("%zu\\n", offsetof({cname}, {c}));' for c in cnames)
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "imop_hip.h"\n'
-                   'int main(void) { printf("%zu\\n", sizeof(imop_hip_frame_desc)); ' + body + ' return 0; }\n')
+                   f'int main(void) {{ printf("%zu\\n", sizeof({cname})); ' + body + ' return 0; }\n')
     exe = tmp_path / "layout"
     subprocess.run([gcc, "-I", HEADER_DIR, str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    want = [ctypes.sizeof(imop_amd.FrameDesc)] + [getattr(imop_amd.FrameDesc, f).offset for f in fields]
+    want = [ctypes.sizeof(ctype)] + [getattr(ctype, f).offset for f in fields]
     assert got == want
 
 
