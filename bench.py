@@ -303,6 +303,137 @@ def bench_single(name, steps, warmup, sets=4):
                          "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
+# Epilogue of the ISP measurement: white balance gains, colour matrix, gamma
+ISP_EPILOGUE = dict(gains=[1.9, 0.8, 2.3], ccm=[1.6, -0.4, -0.2, -0.3, 1.5, -0.2, 0.05, -0.55, 1.5], gamma=2.2)
+
+
+def bench_isp(steps, warmup, with_cpu, sets=6):
+    """Demosaic + RGB epilogue (white balance -> colour matrix -> gamma, the
+    reference's CPU semantics) on the headline frame: fused into one launch
+    (imop_hip_demosaic_isp), against the demosaic followed by the standalone
+    epilogue pass and against the demosaic alone; parity of the fused output
+    against the oracle; each form's `steps` launches captured into a HIP graph
+    and timed after untimed replays."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS["mg12"]
+    blv = [bl] * 4
+    raw = scene.raw_bytes(scene.make_scene(h, w, bd, 0, seed=seed, n_lines=lines), bd, packed)
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(sets)]
+    d_out = [torch.empty(imop_amd.rgb_byte_size(w, h, bd), dtype=torch.uint8, device=dev) for _ in range(sets)]
+    ep = imop_amd.rgb_epilogue(**ISP_EPILOGUE)
+
+    # launches go to torch's current stream (the capture stream inside a graph capture)
+    def fused(k):
+        if imop_amd.demosaic_isp(d_in[k % sets], d_out[k % sets], w, h, 0, algo, bd, packed, blv, epilogue=ep) != 0:
+            raise RuntimeError("imop_hip_demosaic_isp failed")
+
+    def plain(k):
+        if imop_amd.demosaic(d_in[k % sets], d_out[k % sets], w, h, 0, algo, bd, packed, blv) != 0:
+            raise RuntimeError("imop_hip_demosaic failed")
+
+    def split(k):
+        plain(k)
+        if imop_amd.rgb_epilogue_apply(d_out[k % sets], w, h, bd, ep) != 0:
+            raise RuntimeError("imop_hip_rgb_epilogue_apply failed")
+
+    fused(0)  # also builds the gamma table outside any capture
+    torch.cuda.synchronize()
+    got = d_out[0].cpu().numpy().view(np.uint16).reshape(h, w, 3)
+    e, rgb = O.demosaic(raw, w, h, 0, int(algo), bd, packed, blv)
+    want = O.isp_epilogue(rgb, bd, 7, ISP_EPILOGUE["gains"], ISP_EPILOGUE["ccm"], ISP_EPILOGUE["gamma"])
+    parity = "bit-exact vs oracle" if e == 0 and np.array_equal(got, want) else "MISMATCH"
+    split(1)
+    torch.cuda.synchronize()
+    split_ok = np.array_equal(d_out[1].cpu().numpy().view(np.uint16).reshape(h, w, 3), want)
+
+    def graph_ms(fn):
+        for k in range(warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for k in range(steps):
+                fn(k)
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        while time.perf_counter() - t0 < 0.25:
+            g.replay()
+            torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    ms_fused = graph_ms(fused)
+    ms_split = graph_ms(split)
+    ms_plain = graph_ms(plain)
+    log(f"bench: isp fused {ms_fused * 1e3:.1f} us, demosaic + epilogue pass {ms_split * 1e3:.1f} us, "
+        f"demosaic alone {ms_plain * 1e3:.1f} us; parity {parity}")
+    alg = int(raw.size) + imop_amd.rgb_byte_size(w, h, bd)
+    out = {
+        "workload": "MG demosaic 4024x3032 RGGB RAW12-packed, black level 64 -> white balance (1.9, 0.8, 2.3) -> "
+                    "3x3 colour matrix -> gamma 2.2 -> u16 RGB",
+        "value": round(w * h / (ms_fused / 1e3) / 1e6, 1),
+        "unit": "Mpix/s",
+        "ms_per_frame": round(ms_fused, 5),
+        "demosaic_then_epilogue_pass_ms": round(ms_split, 5),
+        "demosaic_alone_ms": round(ms_plain, 5),
+        "speedup_vs_separate_pass": round(ms_split / ms_fused, 3),
+        "parity": parity,
+        "separate_pass_parity": "bit-exact vs oracle" if split_ok else "MISMATCH",
+        "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
+                     "achieved": round(alg / (ms_fused / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (ms_fused / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "steps": steps,
+        "warmup": warmup,
+    }
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_isp(raw, budget_s=6.0)
+    return out
+
+
+def cpu_baseline_isp(raw, budget_s=6.0):
+    """The reference CPU path for the same frame: its demosaic (OpenMP rows)
+    followed by its white balance, colour matrix and gamma passes (single
+    thread, as the reference runs them)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS["mg12"]
+    g, m, gm = ISP_EPILOGUE["gains"], ISP_EPILOGUE["ccm"], ISP_EPILOGUE["gamma"]
+    if O.reference_available() and O.reference_isp_available():
+        kind, threads = "reference", O.reference().ref_num_threads()
+        run = lambda: O.ref_gamma(O.ref_ccm(O.ref_wb(O.ref_demosaic(raw, w, h, 0, algo, bd, packed, [bl] * 4)[1],
+                                                     bd, g), bd, m), bd, gm)
+    else:
+        kind, threads = "port", O.oracle().orc_num_threads()
+        run = lambda: O.isp_epilogue(O.demosaic(raw, w, h, 0, algo, bd, packed, [bl] * 4)[1], bd, 7, g, m, gm)
+    times = []
+    t_end = time.time() + budget_s
+    while len(times) < 2 or (time.time() < t_end and len(times) < 50):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": round(w * h / med / 1e6, 2), "unit": "Mpix/s", "cores": int(threads), "kind": kind,
+            "sample": f"{len(times)} full {w}x{h} frames, median {med * 1e3:.0f} ms (demosaic on {threads} threads, "
+                      "then the three RGB passes on one thread each, as the reference runs them)"}
+
+
 def calibration_copies(mb):
     """4 plain device copies of mb MiB: the known bytes rocprofv3's
     FETCH_SIZE / WRITE_SIZE are scaled against."""
@@ -467,8 +598,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain"), default=None,
-                    help="run only the batch or the chain measurement and print its JSON object (for profiling)")
+    ap.add_argument("--only", choices=("batch", "chain", "isp"), default=None,
+                    help="run only the batch, chain or ISP-epilogue measurement and print its JSON object "
+                         "(for profiling)")
     ap.add_argument("--batch-shape", default=None,
                     help="rows,waves_per_block,blocks_per_cu,order for the batched kernel instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -495,6 +627,11 @@ def main():
         return
     if args.only == "chain":
         out = bench_chain(args.steps, args.warmup, False)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "isp":
+        out = bench_isp(args.steps, args.warmup, False)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -631,6 +768,9 @@ def main():
     if not args.no_chain and args.workload == "mg12":
         torch.cuda.empty_cache()
         result["chain"] = bench_chain(50, 5, not args.no_cpu_baseline)
+    if not args.no_batch and args.workload == "mg12":
+        torch.cuda.empty_cache()
+        result["isp_epilogue"] = bench_isp(100, 5, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 

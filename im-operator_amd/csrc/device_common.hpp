@@ -67,6 +67,7 @@ struct Epilogue {
     float gain[3];         // white balance gains r, g, b
     float m[9];            // colour matrix, row-major, out = M * in
     uint32_t lut_max;      // last gamma table entry: 255 (u8 RGB) or 65535 (u16 RGB)
+    int32_t bounded;       // parameters finite and small: no stage value reaches 2^31
     const uint16_t* lut;   // gamma table for every storable value 0 .. lut_max (device)
 };
 

@@ -9,10 +9,12 @@
 // reference build's operation order with the multiply-adds GCC fuses there
 // (-ffp-contract=fast for C++): fmaf(v, gain, 0.5f) for white balance,
 // fmaf(m2, b, fmaf(m0, r, m1 * g)) then + 0.5f for the matrix.  Gamma is a
-// table over every storable value (256 for u8 RGB, 65536 for u16), built on
-// the host with the same powf the reference calls: the stage is a function
-// of the integer value alone, and values above the bit depth's maximum (u8
-// frames below 8 bits) are mapped as the reference maps them.
+// table over every storable value (256 entries for u8 RGB, 65536 for u16),
+// built on the host with the same powf the reference calls: the stage is a
+// function of the integer value alone, values above the bit depth's maximum
+// included (u8 frames below 8 bits, or any u16 image handed to the
+// standalone pass), which map to the maximum or, where c * max reaches 2^31,
+// to 0 as in the reference.
 #pragma once
 
 #include "device_common.hpp"
@@ -22,34 +24,56 @@ namespace imop_hip {
 // detail::clamp_val(static_cast<int>(v), maxv) as the reference's x86-64
 // build computes it: cvttss2si truncates, and gives INT_MIN (clamped to 0)
 // for NaN and for values of 2^31 or more, where the GPU's conversion would
-// saturate instead.
+// saturate instead.  BOUNDED (the host has checked that no stage value can
+// reach 2^31 and no parameter is NaN or infinite): clamping in float first,
+// then truncating, gives the same integer in two instructions.
+template <bool BOUNDED>
 __device__ __forceinline__ int epi_round_clamp(float v, int maxv) {
+    if (BOUNDED) return (int)__builtin_amdgcn_fmed3f(v, 0.0f, (float)maxv);
     if (!(v < 2147483648.0f)) return 0;
     return imin(imax((int)fmaxf(v, -1.0f), 0), maxv);
 }
 
-__device__ __forceinline__ void epi_px(const Epilogue& e, int& r, int& g, int& b) {
+// One pixel through the stages; `look(v)` is the gamma table lookup.
+template <bool BOUNDED, typename Look>
+__device__ __forceinline__ void epi_px_t(const Epilogue& e, int& r, int& g, int& b, Look look) {
 #pragma clang fp contract(off)
     if (e.flags & EPI_WB) {
-        r = epi_round_clamp(fmaf((float)r, e.gain[0], 0.5f), e.maxv);
-        g = epi_round_clamp(fmaf((float)g, e.gain[1], 0.5f), e.maxv);
-        b = epi_round_clamp(fmaf((float)b, e.gain[2], 0.5f), e.maxv);
+        r = epi_round_clamp<BOUNDED>(fmaf((float)r, e.gain[0], 0.5f), e.maxv);
+        g = epi_round_clamp<BOUNDED>(fmaf((float)g, e.gain[1], 0.5f), e.maxv);
+        b = epi_round_clamp<BOUNDED>(fmaf((float)b, e.gain[2], 0.5f), e.maxv);
     }
     if (e.flags & EPI_CCM) {
         const float fr = (float)r, fg = (float)g, fb = (float)b;
         const float ro = fmaf(e.m[2], fb, fmaf(e.m[0], fr, e.m[1] * fg));
         const float go = fmaf(e.m[5], fb, fmaf(e.m[3], fr, e.m[4] * fg));
         const float bo = fmaf(e.m[8], fb, fmaf(e.m[6], fr, e.m[7] * fg));
-        r = epi_round_clamp(ro + 0.5f, e.maxv);
-        g = epi_round_clamp(go + 0.5f, e.maxv);
-        b = epi_round_clamp(bo + 0.5f, e.maxv);
+        r = epi_round_clamp<BOUNDED>(ro + 0.5f, e.maxv);
+        g = epi_round_clamp<BOUNDED>(go + 0.5f, e.maxv);
+        b = epi_round_clamp<BOUNDED>(bo + 0.5f, e.maxv);
     }
-    if (e.flags & EPI_GAMMA) {  // indices kept inside the table whatever the value
-        r = e.lut[__builtin_elementwise_min((uint32_t)r, e.lut_max)];
-        g = e.lut[__builtin_elementwise_min((uint32_t)g, e.lut_max)];
-        b = e.lut[__builtin_elementwise_min((uint32_t)b, e.lut_max)];
+    if (e.flags & EPI_GAMMA) {
+        r = look(r);
+        g = look(g);
+        b = look(b);
     }
 }
+
+template <typename Look>
+__device__ __forceinline__ void epi_px(const Epilogue& e, int& r, int& g, int& b, Look look) {
+    if (e.bounded) epi_px_t<true>(e, r, g, b, look);
+    else epi_px_t<false>(e, r, g, b, look);
+}
+
+// Table lookups: entries 0 .. n - 1 of `lut` (LDS or global memory), the
+// index clamped into the table whatever the value.
+struct LutRead {
+    const uint16_t* lut;
+    uint32_t last;
+    __device__ __forceinline__ int operator()(int v) const {
+        return lut[__builtin_elementwise_min((uint32_t)v, last)];
+    }
+};
 
 // Standalone form over an interleaved RGB image (u8, or u16 when out16).
 hipError_t launch_rgb_epilogue(uint8_t* rgb, uint64_t npx, int out16, const Epilogue& e, hipStream_t st);

@@ -534,13 +534,19 @@ __device__ __forceinline__ void lane_columns(int x, int w, bool col_ok[4], bool 
     }
 }
 
+// Last gamma table entry the demosaic's output can index: 255 for u8 RGB,
+// the bit depth's maximum for u16 RGB (every output is clamped to it).
+__device__ __forceinline__ uint32_t lut_last(const FastParams& f) {
+    return f.out16 ? (uint32_t)f.maxv : 255u;
+}
+
 // One step: output rows yA, yA + 1 (yA even) of the lane run at x.  Pw holds
 // the decoded rows yA-1 .. yA+4; Dw the MG colour differences of rows yA-1,
 // yA on entry, and the step forms rows yA+1, yA+2 into Dw[2], Dw[3].
 template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD, bool EPI>
 __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, int x, int yA, uint32_t* stage,
                                           const int (*Pw)[4], int (*Dw)[3], const bool col_ok[4],
-                                          const bool col_core[4]) {
+                                          const bool col_core[4], const uint16_t* lut) {
     const int w = f.w, h = f.h;
     const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
     if (MODE != MODE_HQLI) {
@@ -597,7 +603,7 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) epi_px(f.epi, R[a][k], G[a][k], B[a][k]);
+            for (int k = 0; k < 4; ++k) epi_px(f.epi, R[a][k], G[a][k], B[a][k], LutRead{lut, lut_last(f)});
     }
     // w % 4 == 2: column w-3 of the right-edge column fill lives in the
     // lane to the left (DPP while every lane is active)
@@ -664,7 +670,8 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
 // One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
 // Every load of the band is issued up front.
 template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false>
-__device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage) {
+__device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage,
+                                         const uint16_t* lut = nullptr) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
     const int x = sp.xs - 4 + 4 * (int)(threadIdx.x & 63);
     const bool lane_ok = x >= 0 && x < f.w;
@@ -693,7 +700,7 @@ __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, i
         decode(2 * st + 6);
         decode(2 * st + 7);
         band_step<FMT, PAT, MODE, CAREFUL, ODD, EPI>(f, sp, x, y0 + 2 * st, stage, &P[2 * st + 2], &D[2 * st],
-                                                     col_ok, col_core);
+                                                     col_ok, col_core, lut);
     }
 }
 
@@ -726,6 +733,14 @@ __device__ __forceinline__ bool wave_place(const FastParams& f, int& sx, int& by
 template <int FMT, int PAT, int MODE, int TH, bool ODD, bool EPI = false>
 __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
+    // epilogue form: the part of the gamma table the demosaic's values can
+    // reach (lut_last + 1 <= 4096 entries, checked on the host) in LDS,
+    // filled by the whole workgroup before any wave leaves
+    __shared__ uint16_t s_lut[EPI ? 4096 : 1];
+    if (EPI && (f.epi.flags & EPI_GAMMA)) {
+        for (int i = (int)threadIdx.x; i <= (int)lut_last(f); i += (int)blockDim.x) s_lut[i] = f.epi.lut[i];
+        __syncthreads();
+    }
     int sx, by;
     if (!wave_place(f, sx, by)) return;
     const Strip sp = strip_of(f, sx);
@@ -734,8 +749,8 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
     // window reaches the stream's final row
     const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
     uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD, EPI>(f, sp, y0, stage);
-    else run_band<FMT, PAT, MODE, false, TH, ODD, EPI>(f, sp, y0, stage);
+    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD, EPI>(f, sp, y0, stage, s_lut);
+    else run_band<FMT, PAT, MODE, false, TH, ODD, EPI>(f, sp, y0, stage, s_lut);
 }
 
 // ---------------------------------------------------------------- launchers

@@ -141,7 +141,12 @@ hipError_t get_scratch(size_t need, void** out) {
 int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st, const Epilogue* epi) {
     hipError_t he;
     const bool with_epi = epi && epi->flags;
-    if (with_epi && algo != kPrism && fast_path_ok(f)) {
+    // fused into the register-resident kernel, which keeps the part of the
+    // gamma table its outputs can index in LDS (u8 RGB: 256 entries; u16 RGB:
+    // max + 1 <= 4096, bit depths up to 12); otherwise the epilogue runs as
+    // its own pass
+    const uint32_t lds_entries = f.out16 ? (uint32_t)f.maxv + 1u : 256u;
+    if (with_epi && algo != kPrism && fast_path_ok(f) && (!(epi->flags & EPI_GAMMA) || lds_entries <= 4096u)) {
         he = launch_fast(f, mode, st, epi);
         return he == hipSuccess ? kOk : kInternal;
     }

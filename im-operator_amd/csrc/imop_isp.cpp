@@ -90,6 +90,19 @@ int make_epilogue(const imop_hip_rgb_epilogue* p, int bd, Epilogue* e) {
     for (int k = 0; k < 3; ++k) e->gain[k] = p->wb_gains[k];
     for (int k = 0; k < 9; ++k) e->m[k] = p->ccm[k];
     e->lut_max = bd <= 8 ? 255u : 65535u;
+    // every stage value below 2^31 and never NaN: inputs are at most 65535,
+    // so finite parameters with |gain| and each row's sum |m| at most 16384 do
+    bool bounded = true;
+    for (int k = 0; k < 3; ++k) bounded = bounded && std::isfinite(e->gain[k]) && std::fabs(e->gain[k]) <= 16384.0f;
+    for (int r = 0; r < 3; ++r) {
+        float s = 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            bounded = bounded && std::isfinite(e->m[3 * r + c]);
+            s += std::fabs(e->m[3 * r + c]);
+        }
+        bounded = bounded && s <= 16384.0f;
+    }
+    e->bounded = bounded ? 1 : 0;
     if (p->flags & EPI_GAMMA && gamma_lut(bd, p->gamma, &e->lut) != hipSuccess) return kInternal;
     return kOk;
 }

@@ -93,14 +93,34 @@ def test_separate_pass_paths(case):
 
 
 @pytest.mark.parametrize("algo", [A.HQLI, A.MG])
-def test_values_above_the_bit_depth_maximum(algo):
-    """6-bit frames stored as bytes: raw values up to 255 demosaic past 63,
-    and gamma maps them as the reference does (its table covers every byte)."""
-    w, h, bd = 264, 20, 6
-    raw = np.random.default_rng(5).integers(0, 256, size=w * h, dtype=np.uint8)
+@pytest.mark.parametrize("bd,gamma", [(6, 2.2), (4, 0.1), (7, 0.3)])
+def test_values_above_the_bit_depth_maximum(algo, bd, gamma):
+    """Frames below 8 bits stored as bytes: raw values up to 255 demosaic past
+    the maximum, and gamma maps them as the reference does -- to the maximum,
+    or to 0 where powf(v / max, 1 / gamma) * max reaches 2^31 (gamma 0.1 at
+    4 bits)."""
+    w, h = 264, 20
+    raw = np.random.default_rng(5 + bd).integers(0, 256, size=w * h, dtype=np.uint8)
     for flags in (4, 2, 7):
-        got = _run(raw, w, h, 0, algo, bd, False, None, _epi(flags))
-        np.testing.assert_array_equal(got, _oracle(raw, w, h, 0, algo, bd, False, None, flags))
+        got = _run(raw, w, h, 0, algo, bd, False, None, _epi(flags, gamma=gamma))
+        np.testing.assert_array_equal(got, _oracle(raw, w, h, 0, algo, bd, False, None, flags, gamma=gamma))
+
+
+@pytest.mark.parametrize("bd", [10, 12])
+@pytest.mark.parametrize("gamma", [2.2, 0.1])
+def test_standalone_values_above_the_maximum(bd, gamma):
+    """The standalone pass on u16 data over the whole 0..65535 range: values
+    up to the maximum come from the LDS copy of the table, larger ones from
+    the whole table in global memory."""
+    w, h = 256, 64
+    img = np.random.default_rng(bd).integers(0, 65536, size=(h, w, 3)).astype(np.uint16)
+    img[0, :3] = [[(1 << bd) - 1] * 3, [1 << bd] * 3, [65535] * 3]
+    d = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(DEV)
+    ep = _epi(4, gamma=gamma)
+    assert imop_amd.rgb_epilogue_apply(d, w, h, bd, ep) == 0
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().view(np.uint16).reshape(h, w, 3)
+    np.testing.assert_array_equal(got, O.isp_epilogue(img, bd, 4, gamma=gamma))
 
 
 @pytest.mark.parametrize("params", [
