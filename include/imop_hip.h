@@ -213,7 +213,7 @@ typedef struct imop_hip_rgb_epilogue {
  * in-place pass otherwise.  Replaces the reference's demosaic_cuda
  * (src/algorithms/cuda_dispatch.cpp:68-141) followed by
  * process_white_balance_cuda / process_ccm_cuda / process_tone_cuda
- * (src/gpu_dispatch.cpp:292-356), each a host round trip of the RGB image.
+ * (src/gpu_dispatch.cpp:288-360), each a host round trip of the RGB image.
  * epilogue NULL or flags 0: imop_hip_demosaic.  The first call with a given
  * (bit depth, gamma) builds that gamma table on the device with a blocking
  * copy, so it must not be made during stream capture; up to 64 tables are
@@ -226,7 +226,7 @@ int imop_hip_demosaic_isp(const void* d_in, uint64_t in_bytes, void* d_out, uint
 /* The epilogue alone, in place, on an interleaved RGB image of width x
  * height (u8 if bit_depth <= 8, else u16): the reference's
  * process_white_balance_cuda / process_ccm_cuda / process_tone_cuda
- * (src/gpu_dispatch.cpp:292-356) with CPU semantics, device buffer. */
+ * (src/gpu_dispatch.cpp:288-360) with CPU semantics, device buffer. */
 int imop_hip_rgb_epilogue_apply(void* d_rgb, uint64_t rgb_bytes, int width, int height, int bit_depth,
                                 const imop_hip_rgb_epilogue* epilogue, void* stream);
 
