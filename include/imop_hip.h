@@ -92,7 +92,7 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
  * wave order: bit 0 remaps workgroups so each XCD works on a contiguous run
  * of waves (else launch order), bit 1 orders waves strip-fastest (adjacent
  * waves cover adjacent columns of the same rows; else they walk down a
- * 248-column strip).  Out-of-range values leave a setting
+ * strip).  Out-of-range values leave a setting
  * unchanged.  Process-wide; has no counterpart in the reference (its CUDA
  * launch shape is fixed at 32x16, src/algorithms/cuda_kernels.cu:1496). */
 int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
