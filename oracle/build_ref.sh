@@ -9,7 +9,7 @@
 # hosts; neither changes results without -ffast-math.
 #
 # A second library, libimop_ref_isp.so, holds the CPU white balance, colour
-# matrix and gamma stages.  Their module dispatch files (needed for the input
+# matrix, gamma and polynomial lens-shading stages.  Their module dispatch files (needed for the input
 # validation they call) also reference the CUDA entry points, which a build
 # without CUDA does not define; compiling with hidden visibility and one
 # section per function and linking with --gc-sections drops the unused
@@ -57,6 +57,8 @@ ISP_SRCS=(
     "$REF/src/ccm/dispatch.cpp"
     "$REF/src/tone/gamma.cpp"
     "$REF/src/tone/dispatch.cpp"
+    "$REF/src/lens_shading/polynomial.cpp"
+    "$REF/src/lens_shading/dispatch.cpp"
     "$HERE/ref_isp_shim.cpp"
 )
 g++ -std=c++17 -O3 -mavx2 -mfma -fPIC -shared -w -fvisibility=hidden -ffunction-sections -fdata-sections \

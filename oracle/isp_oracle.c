@@ -1,7 +1,8 @@
 /*
  * isp_oracle.c -- CPU restatement of im-operator's integer-domain RGB stages
- * that follow the demosaic: manual white balance, the linear 3x3 colour
- * matrix and gamma.
+ * that follow the demosaic (manual white balance, the linear 3x3 colour
+ * matrix, gamma) and of the polynomial lens-shading correction that runs on
+ * the Bayer frame before it.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP path is compared
  * against; it is never linked into, loaded by, or called from the product
@@ -12,6 +13,8 @@
  *   - manual white balance     src/white_balance/manual.cpp:8-41
  *   - linear 3x3 colour matrix src/ccm/linear_3x3.cpp:9-46
  *   - gamma                    src/tone/gamma.cpp:10-34
+ *   - polynomial lens shading  src/lens_shading/polynomial.cpp:11-52,
+ *                              src/lens_shading/common.hpp:12-35
  *   - helpers                  src/white_balance/common.hpp:13-47 (clamp_val,
  *                              safe_max_val, read_pixel, write_pixel; the ccm
  *                              and tone copies are the same)
@@ -22,15 +25,18 @@
  * explicit fmaf() calls: `r * gain + 0.5f` -> fmaf(r, gain, 0.5f);
  * `m0*r + m1*g + m2*b` -> fmaf(m2, b, fmaf(m0, r, m1 * g)) (GCC fuses the
  * first product of each sum into the addition); `corrected * mv + 0.5f` ->
- * fmaf.  static_cast<int> of a float is cvttss2si on x86-64: truncation,
+ * fmaf; lens shading: `cx*cx + cy*cy` -> fmaf(cx, cx, cy * cy), the gain
+ * polynomial -> fmaf(a6, r6, fmaf(a4, r4, fmaf(a2, r2, 1.0f))),
+ * `v * gain + 0.5f` -> fmaf.  static_cast<int> of a float is cvttss2si on x86-64: truncation,
  * and INT_MIN for NaN or anything outside [-2^31, 2^31).
  *
  * Pinning: tests/test_isp_oracle.py checks every function against the
  * compiled reference stages (oracle/_ref/libimop_ref_isp.so, built from the
  * reference sources by oracle/build_ref.sh) on random images with ordinary
- * and extreme parameters, and against closed forms (unit gains, identity
- * matrix and gamma 1 leave an image unchanged; a diagonal matrix equals the
- * gains that are its diagonal).
+ * and extreme parameters, against closed forms (unit gains, identity matrix
+ * and gamma 1 leave an image unchanged; a permutation matrix permutes the
+ * channels) and against the reference's lens-shading unit tests
+ * (tests/unit/test_lens_shading.cpp:24-84).
  */
 #include <math.h>
 #include <stddef.h>
@@ -107,4 +113,42 @@ void isp_epilogue(uint8_t *rgb, int w, int h, int bd, int flags, const float gai
     if (flags & 1) isp_wb_manual(rgb, rgb, w, h, bd, gains);
     if (flags & 2) isp_ccm_linear(rgb, rgb, w, h, bd, m);
     if (flags & 4) isp_gamma(rgb, rgb, w, h, bd, gamma);
+}
+
+/* lens_shading::detail::safe_max_val (src/lens_shading/common.hpp:12-16) */
+static int ls_max_val(int bd) {
+    if (bd <= 0) return 255;
+    if (bd > 16) return 65535;
+    return (1 << bd) - 1;
+}
+
+/* lens_shading::detail::bayer_color: 0 R, 1 Gr (G in the R row), 2 Gb, 3 B
+ * (src/lens_shading/common.hpp:28-33, PatternOffsets of include/imop/types.hpp:88-101) */
+static int ls_color(int y, int x, int pattern) {
+    static const int off[4][4] = {{0, 0, 1, 1}, {1, 1, 0, 0}, {0, 1, 1, 0}, {1, 0, 0, 1}}; /* r_row r_col b_row b_col */
+    const int *o = off[(pattern >= 0 && pattern < 4) ? pattern : 0];
+    if ((y & 1) == o[0] && (x & 1) == o[1]) return 0;
+    if ((y & 1) == o[2] && (x & 1) == o[3]) return 3;
+    if ((y & 1) == o[0]) return 1;
+    return 2;
+}
+
+/* lens_shading::process_polynomial, in place on an unpacked Bayer frame
+ * (u8 for bd <= 8, else u16).  coef: 4 x {a2, a4, a6} for R, Gr, Gb, B;
+ * center_x / center_y normalised to the frame. */
+void isp_lens_shading_poly(uint8_t *data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
+                           float center_y) {
+    const int mv = ls_max_val(bd);
+    const float cx = center_x * (float)w, cy = center_y * (float)h;
+    const float max_r = sqrtf(fmaf(cx, cx, cy * cy));
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float dx = (float)x - cx, dy = (float)y - cy;
+            const float r = sqrtf(fmaf(dx, dx, dy * dy)) / max_r;
+            const float *c = coef + 3 * ls_color(y, x, pattern);
+            const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
+            const float gain = fmaf(c[2], r6, fmaf(c[1], r4, fmaf(c[0], r2, 1.0f)));
+            const size_t i = (size_t)y * (size_t)w + (size_t)x;
+            wr(data, i, bd, clamp_val(cvt_x86(fmaf((float)rd(data, i, bd), gain, 0.5f)), mv));
+        }
 }

@@ -75,7 +75,8 @@ def oracle() -> ctypes.CDLL:
         lib.isp_ccm_linear.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
         lib.isp_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
         lib.isp_epilogue.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, _fp, ctypes.c_float]
-        for fn in ("isp_wb_manual", "isp_ccm_linear", "isp_gamma", "isp_epilogue"):
+        lib.isp_lens_shading_poly.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_float, ctypes.c_float]
+        for fn in ("isp_wb_manual", "isp_ccm_linear", "isp_gamma", "isp_epilogue", "isp_lens_shading_poly"):
             getattr(lib, fn).restype = None
         lib.isp_gamma_clamp.argtypes = [ctypes.c_float]
         lib.isp_gamma_clamp.restype = ctypes.c_float
@@ -115,6 +116,7 @@ def reference_isp() -> ctypes.CDLL:
         lib.ref_wb_manual.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
         lib.ref_ccm_linear.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
         lib.ref_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
+        lib.ref_lens_shading_poly.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_float, ctypes.c_float]
         _ref_isp = lib
     return _ref_isp
 
@@ -155,6 +157,27 @@ def isp_epilogue(rgb, bd, flags, gains=(1.0, 1.0, 1.0), m=(1, 0, 0, 0, 1, 0, 0, 
                           np.asarray(m, dtype=np.float32).ctypes.data_as(_fp),
                           ctypes.c_float(gamma))
     return out
+
+
+def _lens_shading(lib, fn, bayer, pattern, bd, coef, center):
+    out = np.ascontiguousarray(bayer).copy()
+    h, w = out.shape
+    e = getattr(lib, fn)(out.view(np.uint8).ctypes.data_as(_u8p), w, h, pattern, bd,
+                         np.asarray(coef, dtype=np.float32).ravel().ctypes.data_as(_fp),
+                         ctypes.c_float(center[0]), ctypes.c_float(center[1]))
+    if e not in (None, 0):
+        raise RuntimeError(f"{fn} returned {e}")
+    return out
+
+
+def lens_shading(bayer, pattern, bd, coef, center=(0.5, 0.5)):
+    """lens_shading::process_polynomial (src/lens_shading/polynomial.cpp:11-52),
+    restated; bayer [h, w] u8 / u16, coef 4 x (a2, a4, a6) for R, Gr, Gb, B."""
+    return _lens_shading(oracle(), "isp_lens_shading_poly", bayer, pattern, bd, coef, center)
+
+
+def ref_lens_shading(bayer, pattern, bd, coef, center=(0.5, 0.5)):
+    return _lens_shading(reference_isp(), "ref_lens_shading_poly", bayer, pattern, bd, coef, center)
 
 
 def ref_wb(rgb, bd, gains):

@@ -1,5 +1,5 @@
 // ref_isp_shim.cpp -- C entry points over the reference's own CPU white
-// balance, colour matrix and gamma stages, so tests can call the compiled
+// balance, colour matrix, gamma and lens-shading stages, so tests can call the compiled
 // reference through ctypes.  TEST INFRASTRUCTURE ONLY: built by
 // oracle/build_ref.sh together with the reference sources under
 // /root/reference into oracle/_ref/libimop_ref_isp.so, used to pin
@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "ccm/algorithms.hpp"
+#include "lens_shading/algorithms.hpp"
 #include "tone/algorithms.hpp"
 #include "white_balance/algorithms.hpp"
 
@@ -33,4 +34,20 @@ REF_EXPORT int ref_gamma(const uint8_t* in, uint8_t* out, int w, int h, int bd, 
     tone::ToneParams p;
     p.gamma = gamma;
     return static_cast<int>(tone::process_gamma(in, out, w, h, 3, bd, p));
+}
+
+// lens_shading::process_polynomial, src/lens_shading/polynomial.cpp:11-52
+// (in place; coef = {a2, a4, a6} for R, Gr, Gb, B)
+REF_EXPORT int ref_lens_shading_poly(uint8_t* data, int w, int h, int pattern, int bd, const float* coef, float cx,
+                                     float cy) {
+    lens_shading::LensShadingParams p;
+    lens_shading::ShadingPolynomial* c[4] = {&p.r_coef, &p.gr_coef, &p.gb_coef, &p.b_coef};
+    for (int k = 0; k < 4; ++k) {
+        c[k]->a2 = coef[3 * k];
+        c[k]->a4 = coef[3 * k + 1];
+        c[k]->a6 = coef[3 * k + 2];
+    }
+    p.center_x = cx;
+    p.center_y = cy;
+    return static_cast<int>(lens_shading::process_polynomial(data, w, h, static_cast<imop::BayerPattern>(pattern), bd, p));
 }

@@ -1,7 +1,9 @@
 """The RGB epilogue oracle (oracle/isp_oracle.c: manual white balance, linear
-3x3 colour matrix, gamma) against the compiled reference stages
-(src/white_balance/manual.cpp, src/ccm/linear_3x3.cpp, src/tone/gamma.cpp,
-built by oracle/build_ref.sh) and against closed forms."""
+3x3 colour matrix, gamma) and the polynomial lens-shading oracle against the
+compiled reference stages (src/white_balance/manual.cpp, src/ccm/linear_3x3.cpp,
+src/tone/gamma.cpp, src/lens_shading/polynomial.cpp, built by
+oracle/build_ref.sh), against closed forms and against the reference's
+lens-shading unit tests."""
 from __future__ import annotations
 
 import numpy as np
@@ -106,3 +108,58 @@ def test_planted_mistake_is_detected():
     ref = O.ref_ccm(img, bd, m)
     assert not np.array_equal(naive, ref)
     np.testing.assert_array_equal(O.isp_ccm(img, bd, m), ref)
+
+
+# ------------------------------------------------------------ lens shading
+
+LS_CASES = [([0.8, 0.1, -0.05] * 4, (0.5, 0.5)), ([0.3, 0.0, 0.0, 0.5, 0.1, 0.0, 0.45, 0.1, 0.0, 0.9, -0.2, 0.3], (0.4, 0.6)),
+            ([0.0] * 12, (0.0, 0.0)), ([1e9, 0.0, 0.0] * 4, (0.5, 0.5)), ([-3.0, 1.0, 0.5] * 4, (1.5, -0.25))]
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", [1, 8, 10, 12, 16])
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+@pytest.mark.parametrize("ci", range(len(LS_CASES)))
+def test_lens_shading_matches_reference(bd, pattern, ci):
+    coef, ctr = LS_CASES[ci]
+    rng = np.random.default_rng(bd * 10 + pattern)
+    img = rng.integers(0, 1 << bd, size=(23, 38)).astype(np.uint8 if bd <= 8 else np.uint16)
+    np.testing.assert_array_equal(O.lens_shading(img, pattern, bd, coef, ctr),
+                                  O.ref_lens_shading(img, pattern, bd, coef, ctr))
+
+
+def test_lens_shading_reference_unit_tests():
+    """tests/unit/test_lens_shading.cpp:24-84 of the reference."""
+    # PolynomialIdentity: zero coefficients leave the frame unchanged
+    img = np.full((16, 16), 128, dtype=np.uint8)
+    np.testing.assert_array_equal(O.lens_shading(img, 0, 8, [0.0] * 12), img)
+    # PolynomialBrightensCorners: centre stays ~120, the corner brightens
+    out = O.lens_shading(np.full((100, 100), 120, dtype=np.uint8), 0, 8, [0.8, 0.0, 0.0] * 4)
+    assert abs(int(out[50, 50]) - 120) <= 5 and out[0, 0] > 120
+    # PolynomialRespectsBayerChannels: only R boosted (RGGB: R at (0,0), Gr at (0,1))
+    out = O.lens_shading(np.full((20, 20), 100, dtype=np.uint8), 0, 8, [1.0, 0.0, 0.0] + [0.0] * 9)
+    assert out[0, 0] > 100 and out[0, 1] == 100
+    # OutputInRange (BGGR, i % 256 ramp)
+    img = (np.arange(32 * 32) % 256).astype(np.uint8).reshape(32, 32)
+    out = O.lens_shading(img, 1, 8, [0.5, 0.2, 0.0] * 4)
+    assert out.max() <= 255
+
+
+@needs_ref
+def test_lens_shading_planted_mistake_is_detected():
+    """The unfused float32 gain (products rounded before the sums) differs from
+    the reference somewhere; the fused order the oracle follows matches."""
+    w, h, bd = 256, 192, 16
+    img = np.random.default_rng(3).integers(0, 65536, size=(h, w)).astype(np.uint16)
+    coef = [0.31, 0.17, 0.07] * 4
+    cx, cy = np.float32(0.5 * w), np.float32(0.5 * h)
+    max_r = np.sqrt(cx * cx + cy * cy, dtype=np.float32)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    r = np.sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy), dtype=np.float32) / max_r
+    r2 = r * r
+    a2, a4, a6 = (np.float32(v) for v in coef[:3])
+    gain = np.float32(1) + a2 * r2 + a4 * (r2 * r2) + a6 * (r2 * r2 * r2)
+    naive = np.clip(np.trunc(img.astype(np.float32) * gain + np.float32(0.5)), 0, 65535).astype(np.uint16)
+    ref = O.ref_lens_shading(img, 0, bd, coef)
+    assert not np.array_equal(naive, ref)
+    np.testing.assert_array_equal(O.lens_shading(img, 0, bd, coef), ref)
