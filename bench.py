@@ -434,6 +434,81 @@ def cpu_baseline_isp(raw, budget_s=6.0):
                       "then the three RGB passes on one thread each, as the reference runs them)"}
 
 
+LENS_SHADING = dict(coef=[0.8, 0.1, -0.05, 0.6, 0.05, 0.0, 0.6, 0.05, 0.0, 0.9, 0.1, -0.05], center=(0.5, 0.5))
+
+
+def bench_lens_shading(steps, warmup, with_cpu, sets=6):
+    """Polynomial lens shading in place on an unpacked 4024x3032 12-bit (u16)
+    Bayer frame (imop_hip_lens_shading_poly): parity of one pass against the
+    oracle, then `steps` passes over rotating frames captured into a HIP graph
+    (the passes keep rescaling the same buffers; the work per pass is the
+    same)."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd = 4024, 3032, 12
+    img = np.random.default_rng(12).integers(0, 1 << bd, size=(h, w)).astype(np.uint16)
+    dev = torch.device("cuda:0")
+    d = [torch.from_numpy(img.view(np.uint8).ravel().copy()).to(dev) for _ in range(sets)]
+    p = imop_amd.lens_shading_params(LENS_SHADING["coef"], LENS_SHADING["center"])
+
+    def run(k):
+        if imop_amd.lens_shading(d[k % sets], w, h, 0, bd, p) != 0:
+            raise RuntimeError("imop_hip_lens_shading_poly failed")
+
+    run(0)
+    torch.cuda.synchronize()
+    want = O.lens_shading(img, 0, bd, LENS_SHADING["coef"], LENS_SHADING["center"])
+    parity = "bit-exact vs oracle" if np.array_equal(d[0].cpu().numpy().view(np.uint16).reshape(h, w), want) \
+        else "MISMATCH"
+    for k in range(warmup):
+        run(k)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for k in range(steps):
+            run(k)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.25:
+        g.replay()
+        torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    alg = 2 * w * h * 2
+    log(f"bench: lens shading {ms * 1e3:.1f} us per frame, parity {parity}")
+    out = {"workload": "polynomial lens shading in place, 4024x3032 RGGB 12-bit u16 Bayer",
+           "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s", "ms_per_frame": round(ms, 5),
+           "parity": parity,
+           "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
+                        "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "steps": steps, "warmup": warmup}
+    if with_cpu:
+        fn, kind = (O.ref_lens_shading, "reference") if O.reference_isp_available() else (O.lens_shading, "port")
+        times = []
+        t_end = time.time() + 4.0
+        while len(times) < 2 or (time.time() < t_end and len(times) < 50):
+            t1 = time.perf_counter()
+            fn(img, 0, bd, LENS_SHADING["coef"], LENS_SHADING["center"])
+            times.append(time.perf_counter() - t1)
+        times.sort()
+        med = times[len(times) // 2]
+        out["cpu_baseline"] = {"value": round(w * h / med / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": kind,
+                               "sample": f"{len(times)} full {w}x{h} frames, median {med * 1e3:.0f} ms "
+                                         "(the reference's CPU loop, one thread)"}
+    return out
+
+
 def calibration_copies(mb):
     """4 plain device copies of mb MiB: the known bytes rocprofv3's
     FETCH_SIZE / WRITE_SIZE are scaled against."""
@@ -598,9 +673,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp"), default=None,
-                    help="run only the batch, chain or ISP-epilogue measurement and print its JSON object "
-                         "(for profiling)")
+    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens"), default=None,
+                    help="run only the batch, chain, ISP-epilogue or lens-shading measurement and print its JSON "
+                         "object (for profiling)")
     ap.add_argument("--batch-shape", default=None,
                     help="rows,waves_per_block,blocks_per_cu,order for the batched kernel instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -632,6 +707,11 @@ def main():
         return
     if args.only == "isp":
         out = bench_isp(args.steps, args.warmup, False)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "lens":
+        out = bench_lens_shading(args.steps, args.warmup, False)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -771,6 +851,8 @@ def main():
     if not args.no_batch and args.workload == "mg12":
         torch.cuda.empty_cache()
         result["isp_epilogue"] = bench_isp(100, 5, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        result["lens_shading"] = bench_lens_shading(100, 5, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 

@@ -78,4 +78,9 @@ struct LutRead {
 // Standalone form over an interleaved RGB image (u8, or u16 when out16).
 hipError_t launch_rgb_epilogue(uint8_t* rgb, uint64_t npx, int out16, const Epilogue& e, hipStream_t st);
 
+// Polynomial lens shading in place on an unpacked Bayer frame (u8 for
+// bd <= 8, else u16); coef = {a2, a4, a6} for R, Gr, Gb, B.
+hipError_t launch_lens_shading(uint8_t* data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
+                               float center_y, hipStream_t st);
+
 }  // namespace imop_hip

@@ -1,6 +1,7 @@
-// imop_isp.cpp -- C ABI of the RGB epilogue (include/imop_hip.h): parameter
-// checks, the per-device gamma tables, the demosaic + epilogue entry point
-// and the standalone in-place pass.
+// imop_isp.cpp -- C ABI of the RGB epilogue and of lens shading
+// (include/imop_hip.h): parameter checks, the per-device gamma tables, the
+// demosaic + epilogue entry point, the standalone in-place epilogue pass and
+// the in-place lens-shading pass.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -143,6 +144,21 @@ int imop_hip_rgb_epilogue_apply(void* d_rgb, uint64_t rgb_bytes_, int width, int
     if (e != kOk) return e;
     return launch_rgb_epilogue((uint8_t*)d_rgb, (uint64_t)width * (uint64_t)height, bit_depth > 8, ep,
                                (hipStream_t)stream) == hipSuccess
+               ? kOk
+               : kInternal;
+}
+
+int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
+                               const imop_hip_lens_shading* params, void* stream) {
+    if (device_count_cached() == 0) return kInternal;
+    // lens_shading::validate_lens_shading_inputs (src/lens_shading/dispatch.cpp:7-17)
+    if (!d_data || !params) return kNull;
+    if (width <= 0 || height <= 0) return kDims;
+    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    const uint64_t need = (uint64_t)width * (uint64_t)height * (bit_depth > 8 ? 2u : 1u);
+    if (bytes < need || (bit_depth > 8 && ((uintptr_t)d_data & 1))) return kInternal;
+    return launch_lens_shading((uint8_t*)d_data, width, height, pattern, bit_depth, &params->coef[0][0],
+                               params->center_x, params->center_y, (hipStream_t)stream) == hipSuccess
                ? kOk
                : kInternal;
 }

@@ -1,8 +1,9 @@
-// isp_kernels.hip -- the RGB epilogue (white balance, colour matrix, gamma;
-// epilogue.hpp) as its own in-place pass over an interleaved RGB image.  The
-// demosaic paths that cannot apply it in registers (PRISM, the generic
-// LDS-tiled kernels) run it after the demosaic; it is also the standalone
-// imop_hip_rgb_epilogue.
+// isp_kernels.hip -- in-place passes around the demosaic:
+//  * the RGB epilogue (white balance, colour matrix, gamma; epilogue.hpp) over
+//    an interleaved RGB image.  The demosaic paths that cannot apply it in
+//    registers (PRISM, the generic LDS-tiled kernels) run it after the
+//    demosaic; it is also the standalone imop_hip_rgb_epilogue_apply;
+//  * the polynomial lens-shading correction over an unpacked Bayer frame.
 #include "epilogue.hpp"
 
 namespace imop_hip {
@@ -81,7 +82,70 @@ __global__ __launch_bounds__(256) void k_rgb_epilogue(uint8_t* __restrict__ rgb,
     }
 }
 
+// Polynomial lens shading (lens_shading::process_polynomial,
+// src/lens_shading/polynomial.cpp:11-52) in place, four samples of a row per
+// thread.  cx, cy and max_r are the reference's per-call values, computed on
+// the host with the same float operations.  Per sample, in the reference
+// build's order with its fused multiply-adds: r = sqrt(fma(dx, dx, dy * dy))
+// / max_r (correctly rounded), gain = fma(a6, r^6, fma(a4, r^4, fma(a2, r^2,
+// 1))), v' = clamp_val(int(fma(v, gain, 0.5)), max) with the x86 conversion.
+struct LensShading {
+    float coef[4][3];   // {a2, a4, a6} for R, Gr, Gb, B
+    float cx, cy, max_r;
+    int32_t maxv;
+    int32_t r_row, r_col, b_row, b_col;  // PatternOffsets
+};
+
+template <int U16>
+__global__ __launch_bounds__(256) void k_lens_shading(uint8_t* __restrict__ data, int w, int h, LensShading p) {
+#pragma clang fp contract(off)
+    const int x0 = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (x0 >= w) return;
+    for (int y = (int)blockIdx.y; y < h; y += (int)gridDim.y) {
+        const float dy = (float)y - p.cy;
+        const float dyy = dy * dy;
+        const bool r_row = (y & 1) == p.r_row, b_row = (y & 1) == p.b_row;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + k;
+            if (x >= w) break;
+            const float dx = (float)x - p.cx;
+            const float r = sqrtf(fmaf(dx, dx, dyy)) / p.max_r;  // both correctly rounded (hipcc default)
+            // lens_shading::detail::bayer_color: 0 R, 1 Gr, 2 Gb, 3 B
+            const int c = (r_row && (x & 1) == p.r_col) ? 0 : (b_row && (x & 1) == p.b_col) ? 3 : r_row ? 1 : 2;
+            const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
+            const float gain = fmaf(p.coef[c][2], r6, fmaf(p.coef[c][1], r4, fmaf(p.coef[c][0], r2, 1.0f)));
+            const uint64_t i = (uint64_t)y * (uint64_t)w + (uint64_t)x;
+            const int v = U16 ? ((const uint16_t*)data)[i] : data[i];
+            const int o = epi_round_clamp<false>(fmaf((float)v, gain, 0.5f), p.maxv);
+            if (U16) ((uint16_t*)data)[i] = (uint16_t)o;
+            else data[i] = (uint8_t)o;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_lens_shading(uint8_t* data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
+                               float center_y, hipStream_t st) {
+#pragma clang fp contract(off)
+    LensShading p;
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 3; ++k) p.coef[c][k] = coef[3 * c + k];
+    // the reference's per-call constants (polynomial.cpp:26-28), same operations
+    p.cx = center_x * (float)w;
+    p.cy = center_y * (float)h;
+    p.max_r = sqrtf(fmaf(p.cx, p.cx, p.cy * p.cy));
+    p.maxv = bd <= 0 ? 255 : bd > 16 ? 65535 : (1 << bd) - 1;  // lens_shading::detail::safe_max_val
+    static const int off[4][4] = {{0, 0, 1, 1}, {1, 1, 0, 0}, {0, 1, 1, 0}, {1, 0, 0, 1}};  // PatternOffsets
+    const int* o = off[(pattern >= 0 && pattern < 4) ? pattern : 0];
+    p.r_row = o[0]; p.r_col = o[1]; p.b_row = o[2]; p.b_col = o[3];
+    const unsigned bx = (unsigned)((w + 1023) / 1024);
+    const dim3 grid(bx, (unsigned)(h < 65535 ? h : 65535)), block(256);
+    if (bd > 8) hipLaunchKernelGGL((k_lens_shading<1>), grid, block, 0, st, data, w, h, p);
+    else hipLaunchKernelGGL((k_lens_shading<0>), grid, block, 0, st, data, w, h, p);
+    return hipGetLastError();
+}
 
 hipError_t launch_rgb_epilogue(uint8_t* rgb, uint64_t npx, int out16, const Epilogue& e, hipStream_t st) {
     if (npx == 0 || e.flags == 0) return hipSuccess;

@@ -114,6 +114,26 @@ def rgb_epilogue(gains=None, ccm=None, gamma=None) -> RgbEpilogue:
     return e
 
 
+class LensShadingParams(ctypes.Structure):
+    """imop_hip_lens_shading (include/imop_hip.h): {a2, a4, a6} for R, Gr, Gb, B
+    and the normalised optical centre."""
+    _fields_ = [
+        ("coef", (ctypes.c_float * 3) * 4),
+        ("center_x", ctypes.c_float),
+        ("center_y", ctypes.c_float),
+    ]
+
+
+def lens_shading_params(coef, center=(0.5, 0.5)) -> LensShadingParams:
+    p = LensShadingParams()
+    c = np.asarray(coef, dtype=np.float64).reshape(4, 3)
+    for i in range(4):
+        for k in range(3):
+            p.coef[i][k] = float(c[i, k])
+    p.center_x, p.center_y = float(center[0]), float(center[1])
+    return p
+
+
 _lib = None
 
 
@@ -153,6 +173,8 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_demosaic_isp.restype = i
         L.imop_hip_rgb_epilogue_apply.argtypes = [vp, u64, i, i, i, ctypes.POINTER(RgbEpilogue), vp]
         L.imop_hip_rgb_epilogue_apply.restype = i
+        L.imop_hip_lens_shading_poly.argtypes = [vp, u64, i, i, i, i, ctypes.POINTER(LensShadingParams), vp]
+        L.imop_hip_lens_shading_poly.restype = i
         L.imop_hip_set_batch_shape.argtypes = [i, i, i, i]
         L.imop_hip_set_batch_shape.restype = i
         L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
@@ -243,6 +265,15 @@ def rgb_epilogue_apply(d_rgb, width: int, height: int, bit_depth: int, epilogue:
     return lib().imop_hip_rgb_epilogue_apply(ctypes.c_void_p(d_rgb.data_ptr()), d_rgb.numel() * d_rgb.element_size(),
                                              width, height, bit_depth, ctypes.byref(epilogue),
                                              ctypes.c_void_p(_stream_arg(stream)))
+
+
+def lens_shading(d_bayer, width: int, height: int, pattern: int, bit_depth: int, params: LensShadingParams,
+                 stream=None) -> int:
+    """imop_hip_lens_shading_poly: polynomial lens shading in place on an unpacked
+    Bayer frame (device tensor, u8 or u16 samples)."""
+    return lib().imop_hip_lens_shading_poly(ctypes.c_void_p(d_bayer.data_ptr()),
+                                            d_bayer.numel() * d_bayer.element_size(), width, height, int(pattern),
+                                            bit_depth, ctypes.byref(params), ctypes.c_void_p(_stream_arg(stream)))
 
 
 def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,

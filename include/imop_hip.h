@@ -230,6 +230,28 @@ int imop_hip_demosaic_isp(const void* d_in, uint64_t in_bytes, void* d_out, uint
 int imop_hip_rgb_epilogue_apply(void* d_rgb, uint64_t rgb_bytes, int width, int height, int bit_depth,
                                 const imop_hip_rgb_epilogue* epilogue, void* stream);
 
+/* ------------------------------------------------------ lens shading */
+
+/* lens_shading::LensShadingParams of the polynomial correction: coef[c] =
+ * {a2, a4, a6} of ShadingPolynomial for c = R, Gr, Gb, B (r_coef, gr_coef,
+ * gb_coef, b_coef); center_x / center_y the normalised optical centre. */
+typedef struct imop_hip_lens_shading {
+    float coef[4][3];
+    float center_x, center_y;
+} imop_hip_lens_shading;
+
+/* Polynomial lens shading in place on an unpacked Bayer frame in device
+ * memory (u8 for bit_depth <= 8, else u16; bytes >= width * height * 1 or 2):
+ * gain = 1 + a2 r^2 + a4 r^4 + a6 r^6 of the radius from the optical centre
+ * normalised by the centre's distance to the origin, per CFA channel.
+ * Replaces lens_shading::process_lens_shading_cuda (src/gpu_dispatch.cpp:
+ * 455-490, POLYNOMIAL) with the CPU semantics of process_polynomial
+ * (src/lens_shading/polynomial.cpp:11-52): same bytes.  Returns a
+ * lens_shading::LensShadingError value (same numbering as DemosaicError;
+ * 4 = InvalidChannels unused). */
+int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
+                               const imop_hip_lens_shading* params, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

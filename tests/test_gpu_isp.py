@@ -2,8 +2,11 @@
 against the oracle: the demosaic oracle followed by the restated white balance,
 colour matrix and gamma (oracle/isp_oracle.c, pinned against the reference's
 compiled CPU stages).  Fused into the register-resident kernel (HQLI / MG of
-even widths) and as a separate pass (PRISM, odd widths, other layouts)."""
+even widths) and as a separate pass (PRISM, odd widths, other layouts).  Also
+the in-place polynomial lens-shading pass (imop_hip_lens_shading_poly)."""
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 import pytest
@@ -194,3 +197,57 @@ def test_graph_replays_match_eager():
         g.replay()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(d_out.cpu().numpy().view(np.uint16).reshape(h, w, 3), want)
+
+
+# ------------------------------------------------------------ lens shading
+
+LS = [([0.8, 0.1, -0.05] * 4, (0.5, 0.5)),
+      ([0.3, 0.0, 0.0, 0.5, 0.1, 0.0, 0.45, 0.1, 0.0, 0.9, -0.2, 0.3], (0.4, 0.6)),
+      ([0.0] * 12, (0.0, 0.0)),            # max_r = 0: the reference divides by zero
+      ([1e9, 0.0, 0.0] * 4, (0.5, 0.5)),   # gains past 2^31
+      ([-3.0, 1.0, 0.5] * 4, (1.5, -0.25))]
+
+
+def _ls_run(img, pattern, bd, coef, center):
+    h, w = img.shape
+    d = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(DEV)
+    assert imop_amd.lens_shading(d, w, h, pattern, bd, imop_amd.lens_shading_params(coef, center)) == 0
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    return (out.view(np.uint16) if bd > 8 else out).reshape(h, w)
+
+
+@pytest.mark.parametrize("bd", [1, 8, 10, 12, 16])
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+@pytest.mark.parametrize("ci", range(len(LS)))
+def test_lens_shading_matches_oracle(bd, pattern, ci):
+    coef, ctr = LS[ci]
+    img = np.random.default_rng(bd * 7 + pattern).integers(0, 1 << bd, size=(37, 131)).astype(
+        np.uint8 if bd <= 8 else np.uint16)
+    np.testing.assert_array_equal(_ls_run(img, pattern, bd, coef, ctr), O.lens_shading(img, pattern, bd, coef, ctr))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 2), (4096 + 3, 5)])
+def test_lens_shading_shapes(w, h):
+    img = np.random.default_rng(w).integers(0, 4096, size=(h, w)).astype(np.uint16)
+    coef, ctr = LS[1]
+    np.testing.assert_array_equal(_ls_run(img, 2, 12, coef, ctr), O.lens_shading(img, 2, 12, coef, ctr))
+
+
+def test_lens_shading_full_frame():
+    w, h = 4024, 3032
+    img = np.random.default_rng(9).integers(0, 4096, size=(h, w)).astype(np.uint16)
+    coef, ctr = LS[0]
+    print("lens shading full frame", flush=True)
+    np.testing.assert_array_equal(_ls_run(img, 0, 12, coef, ctr), O.lens_shading(img, 0, 12, coef, ctr))
+
+
+def test_lens_shading_invalid_arguments():
+    d = torch.zeros(64 * 64 * 2, dtype=torch.uint8, device=DEV)
+    p = imop_amd.lens_shading_params(LS[0][0])
+    L = imop_amd.lib()
+    assert L.imop_hip_lens_shading_poly(None, 0, 64, 64, 0, 12, ctypes.byref(p), None) == 1
+    assert imop_amd.lens_shading(d, 0, 64, 0, 12, p) == 2
+    assert imop_amd.lens_shading(d, 64, 64, 0, 17, p) == 3
+    assert imop_amd.lens_shading(d, 64, 65, 0, 12, p) == 6   # buffer too small
+    assert not d.any()
