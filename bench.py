@@ -509,6 +509,101 @@ def bench_lens_shading(steps, warmup, with_cpu, sets=6):
     return out
 
 
+def _defect_frame(kind, w, h, bd, seed):
+    """A 12-bit test frame: a smooth scene with 0.1 % hot and 0.05 % cold
+    pixels ("sparse"), or pure noise ("noise": corrections chained across the
+    frame, the stage's worst case)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    mx = (1 << bd) - 1
+    if kind == "noise":
+        return rng.integers(0, mx + 1, size=(h, w)).astype(np.uint16)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = mx // 4 + (xx + 2 * yy) % (mx // 8) + rng.integers(0, mx // 64 + 1, size=(h, w))
+    img[rng.random((h, w)) < 0.001] = mx
+    img[rng.random((h, w)) < 0.0005] = 0
+    return img.astype(np.uint16)
+
+
+def bench_defect(steps, warmup, with_cpu):
+    """Adaptive defect correction in place on a 4024x3032 12-bit (u16) Bayer
+    frame (imop_hip_defect_adaptive, threshold 0.3): parity against the oracle,
+    then `steps` calls, each on a fresh copy of the frame (restored outside the
+    timed region).  The call synchronises its stream (the number of
+    fixed-point passes depends on the frame), so a call is timed on the host
+    clock, host round trips included; HIP events on the stream give the GPU
+    span.  Also the worst case: pure noise at threshold 0.05."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd = 4024, 3032, 12
+    dev = torch.device("cuda:0")
+    out = {"workload": "adaptive defect correction in place, 4024x3032 12-bit u16 Bayer, 0.1 % hot + 0.05 % cold "
+                       "pixels, threshold 0.3"}
+    for kind, thr, n in (("sparse", 0.3, steps), ("noise", 0.05, max(3, steps // 10))):
+        img = _defect_frame(kind, w, h, bd, 21)
+        pristine = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(dev)
+        d = pristine.clone()
+        if imop_amd.defect_correct(d, w, h, 0, bd, thr) != 0:
+            raise RuntimeError("imop_hip_defect_adaptive failed")
+        want = O.defect_adaptive(img, bd, thr)[1]
+        parity = "bit-exact vs oracle" if np.array_equal(d.cpu().numpy().view(np.uint16).reshape(h, w), want) \
+            else "MISMATCH"
+        wall, span = [], []
+        for k in range(warmup + n):
+            d.copy_(pristine)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            a.record()
+            if imop_amd.defect_correct(d, w, h, 0, bd, thr) != 0:
+                raise RuntimeError("imop_hip_defect_adaptive failed")
+            b.record()
+            t1 = time.perf_counter()
+            torch.cuda.synchronize()
+            if k >= warmup:
+                wall.append(t1 - t0)
+                span.append(a.elapsed_time(b) / 1e3)
+        wall.sort()
+        span.sort()
+        ms = wall[len(wall) // 2] * 1e3
+        gpu_ms = span[len(span) // 2] * 1e3
+        log(f"bench: defect correction ({kind}) {ms * 1e3:.1f} us per frame (GPU span {gpu_ms * 1e3:.1f} us), "
+            f"parity {parity}")
+        if kind == "sparse":
+            alg = w * h * 2   # the frame read once (corrected pixels written: negligible)
+            out.update({"value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                        "ms_per_frame": round(ms, 5), "gpu_span_ms": round(gpu_ms, 5), "parity": parity,
+                        "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
+                                     "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+                        "steps": n, "warmup": warmup})
+        else:
+            out["worst_case_noise"] = {"threshold": thr, "ms_per_frame": round(ms, 5),
+                                       "gpu_span_ms": round(gpu_ms, 5), "parity": parity, "steps": n}
+        if with_cpu and kind == "sparse":
+            fn, ck = (O.ref_defect_adaptive, "reference") if O.reference_isp_available() else (O.defect_adaptive,
+                                                                                               "port")
+            times = []
+            t_end = time.time() + 6.0
+            while len(times) < 2 or (time.time() < t_end and len(times) < 20):
+                t1 = time.perf_counter()
+                fn(img, bd, thr)
+                times.append(time.perf_counter() - t1)
+            times.sort()
+            med = times[len(times) // 2]
+            out["cpu_baseline"] = {"value": round(w * h / med / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": ck,
+                                   "sample": f"{len(times)} full {w}x{h} frames, median {med * 1e3:.0f} ms "
+                                             "(the reference's CPU loop, one thread)"}
+    return out
+
+
 def calibration_copies(mb):
     """4 plain device copies of mb MiB: the known bytes rocprofv3's
     FETCH_SIZE / WRITE_SIZE are scaled against."""
@@ -673,9 +768,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens"), default=None,
-                    help="run only the batch, chain, ISP-epilogue or lens-shading measurement and print its JSON "
-                         "object (for profiling)")
+    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect"), default=None,
+                    help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
+                         "and print its JSON object (for profiling)")
     ap.add_argument("--batch-shape", default=None,
                     help="rows,waves_per_block,blocks_per_cu,order for the batched kernel instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -713,6 +808,10 @@ def main():
     if args.only == "lens":
         out = bench_lens_shading(args.steps, args.warmup, False)
         calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "defect":
+        out = bench_defect(args.steps, args.warmup, False)
         print(json.dumps(out), flush=True)
         return
 
@@ -853,6 +952,8 @@ def main():
         result["isp_epilogue"] = bench_isp(100, 5, not args.no_cpu_baseline)
         torch.cuda.empty_cache()
         result["lens_shading"] = bench_lens_shading(100, 5, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        result["defect_correct"] = bench_defect(30, 3, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 

@@ -83,4 +83,13 @@ hipError_t launch_rgb_epilogue(uint8_t* rgb, uint64_t npx, int out16, const Epil
 hipError_t launch_lens_shading(uint8_t* data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
                                float center_y, hipStream_t st);
 
+// Adaptive defect correction (defect_kernels.hip): the copy of the input
+// into `dst` (also clearing ctl[0..3]) and one fixed-point pass; tiles of
+// 128 x 32 pixels, `flags` holds 2 bytes per tile.
+hipError_t launch_defect_copy(const uint8_t* src, uint8_t* dst, uint64_t nbytes, int* ctl, hipStream_t st);
+hipError_t launch_defect_pass(uint8_t* data, const uint8_t* orig, uint8_t* flags, int* ctl, int w, int h, int bd,
+                              int thresh, int pass, hipStream_t st);
+int defect_tiles_x(int w);
+int defect_tiles_y(int h);
+
 }  // namespace imop_hip

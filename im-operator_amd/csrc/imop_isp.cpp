@@ -1,11 +1,14 @@
-// imop_isp.cpp -- C ABI of the RGB epilogue and of lens shading
-// (include/imop_hip.h): parameter checks, the per-device gamma tables, the
-// demosaic + epilogue entry point, the standalone in-place epilogue pass and
-// the in-place lens-shading pass.
+// imop_isp.cpp -- C ABI of the RGB epilogue, lens shading and adaptive defect
+// correction (include/imop_hip.h): parameter checks, the per-device gamma
+// tables, the demosaic + epilogue entry point, the standalone in-place
+// epilogue pass, the in-place lens-shading pass and the fixed-point passes of
+// defect correction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -108,6 +111,76 @@ int make_epilogue(const imop_hip_rgb_epilogue* p, int bd, Epilogue* e) {
     return kOk;
 }
 
+// Per-device state of defect correction: the scratch frame (input copy,
+// pass control words, per-tile flags), grown as needed, and a pinned word
+// block for reading the control words back.  One call at a time per device.
+struct DefectState {
+    std::mutex mu;
+    void* d = nullptr;
+    size_t bytes = 0;
+    int* host = nullptr;
+};
+std::mutex g_defect_mu;
+std::vector<std::unique_ptr<DefectState>> g_defect;
+
+DefectState* defect_state(int dev) {
+    std::lock_guard<std::mutex> lk(g_defect_mu);
+    if ((int)g_defect.size() <= dev) g_defect.resize(dev + 1);
+    if (!g_defect[dev]) g_defect[dev].reset(new DefectState);
+    return g_defect[dev].get();
+}
+
+int defect_adaptive(uint8_t* data, int w, int h, int bd, float threshold, hipStream_t st) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return kInternal;
+    DefectState* S = defect_state(dev);
+    std::lock_guard<std::mutex> lk(S->mu);
+    const size_t frame = (size_t)w * (size_t)h * (bd > 8 ? 2u : 1u);
+    const size_t frame_al = (frame + 255u) & ~(size_t)255u;
+    const int ntiles = defect_tiles_x(w) * defect_tiles_y(h);
+    const size_t need = frame_al + 256u + 2u * (size_t)ntiles;
+    if (S->bytes < need) {
+        if (S->d) {
+            if (hipDeviceSynchronize() != hipSuccess) return kInternal;
+            (void)hipFree(S->d);
+            S->d = nullptr;
+            S->bytes = 0;
+        }
+        if (hipMalloc(&S->d, need) != hipSuccess) return kInternal;
+        S->bytes = need;
+    }
+    if (!S->host && hipHostMalloc((void**)&S->host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        S->host = nullptr;
+        return kInternal;
+    }
+    uint8_t* orig = (uint8_t*)S->d;
+    int* ctl = (int*)(orig + frame_al);
+    uint8_t* flags = orig + frame_al + 256u;
+    // std::max(0.05f, std::min(1.0f, threshold)) (adaptive.cpp:17-18; NaN -> 1)
+    const float t_lo = (threshold < 1.0f) ? threshold : 1.0f;
+    const float t = (0.05f < t_lo) ? t_lo : 0.05f;
+    const int mv = bd <= 0 ? 255 : bd > 16 ? 65535 : (int)((1u << bd) - 1u);  // detail::safe_max_val
+    const int thresh = (int)(t * (float)mv);
+    if (launch_defect_copy(data, orig, frame, ctl, st) != hipSuccess) return kInternal;
+    // A pass fixes at least the next pixel of every chain of dependencies, and
+    // the longest chain on a colour plane of pw x ph pixels is pw + 2 ph long.
+    const long pw = (w - 4 + 1) / 2, ph = (h - 4 + 1) / 2;
+    const long cap = pw + 2 * ph + 4;
+    int pass = 0, chunk = 2;
+    for (;;) {
+        for (int i = 0; i < chunk; ++i, ++pass)
+            if (launch_defect_pass(data, orig, flags, ctl, w, h, bd, thresh, pass, st) != hipSuccess)
+                return kInternal;
+        if (hipMemcpyAsync(S->host, ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return kInternal;
+        if (S->host[3]) return kInternal;
+        if (S->host[(pass - 1) % 3] == 0) return kOk;
+        if (pass >= cap) return kInternal;
+        chunk = std::min(chunk * 2, 64);
+    }
+}
+
 }  // namespace
 }  // namespace imop_hip
 
@@ -161,6 +234,21 @@ int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int heig
                                params->center_x, params->center_y, (hipStream_t)stream) == hipSuccess
                ? kOk
                : kInternal;
+}
+
+int imop_hip_defect_adaptive(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
+                             float threshold, void* stream) {
+    (void)pattern;  // same-colour neighbours at distance 2 whatever the CFA order
+    if (device_count_cached() == 0) return kInternal;
+    // defect_correct::validate_defect_correct_inputs (src/defect_correct/dispatch.cpp:7-15)
+    if (!d_data) return kNull;
+    if (width <= 0 || height <= 0) return kDims;
+    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    // process_adaptive (adaptive.cpp:13)
+    if (width < 5 || height < 5) return kSmall;
+    const uint64_t need = (uint64_t)width * (uint64_t)height * (bit_depth > 8 ? 2u : 1u);
+    if (bytes < need || (bit_depth > 8 && ((uintptr_t)d_data & 1))) return kInternal;
+    return defect_adaptive((uint8_t*)d_data, width, height, bit_depth, threshold, (hipStream_t)stream);
 }
 
 }  // extern "C"

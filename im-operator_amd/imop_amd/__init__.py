@@ -175,6 +175,8 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_rgb_epilogue_apply.restype = i
         L.imop_hip_lens_shading_poly.argtypes = [vp, u64, i, i, i, i, ctypes.POINTER(LensShadingParams), vp]
         L.imop_hip_lens_shading_poly.restype = i
+        L.imop_hip_defect_adaptive.argtypes = [vp, u64, i, i, i, i, ctypes.c_float, vp]
+        L.imop_hip_defect_adaptive.restype = i
         L.imop_hip_set_batch_shape.argtypes = [i, i, i, i]
         L.imop_hip_set_batch_shape.restype = i
         L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
@@ -274,6 +276,16 @@ def lens_shading(d_bayer, width: int, height: int, pattern: int, bit_depth: int,
     return lib().imop_hip_lens_shading_poly(ctypes.c_void_p(d_bayer.data_ptr()),
                                             d_bayer.numel() * d_bayer.element_size(), width, height, int(pattern),
                                             bit_depth, ctypes.byref(params), ctypes.c_void_p(_stream_arg(stream)))
+
+
+def defect_correct(d_bayer, width: int, height: int, pattern: int, bit_depth: int, threshold: float = 0.3,
+                   stream=None) -> int:
+    """imop_hip_defect_adaptive: adaptive defect correction in place on an
+    unpacked Bayer frame (device tensor, u8 or u16 samples); threshold as
+    DefectCorrectParams::threshold (default 0.3).  Synchronises the stream."""
+    return lib().imop_hip_defect_adaptive(ctypes.c_void_p(d_bayer.data_ptr()),
+                                          d_bayer.numel() * d_bayer.element_size(), width, height, int(pattern),
+                                          bit_depth, ctypes.c_float(threshold), ctypes.c_void_p(_stream_arg(stream)))
 
 
 def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,

@@ -252,6 +252,26 @@ typedef struct imop_hip_lens_shading {
 int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
                                const imop_hip_lens_shading* params, void* stream);
 
+/* ---------------------------------------------- adaptive defect correction */
+
+/* Adaptive defect correction in place on an unpacked Bayer frame in device
+ * memory (u8 for bit_depth <= 8, else u16; bytes >= width * height * 1 or 2):
+ * each interior pixel ([2, w-2) x [2, h-2)) whose distance from the median of
+ * its same-colour neighbours at distance 2 exceeds
+ * clamp(threshold, 0.05, 1) * max becomes that median, in the reference's
+ * raster order (neighbours above and to the left as already corrected).
+ * Replaces defect_correct::process_defect_correct_cuda (src/gpu_dispatch.cpp:
+ * 529-571, ADAPTIVE) with the CPU semantics of process_adaptive
+ * (src/defect_correct/adaptive.cpp:8-61): same bytes.  `pattern` is accepted
+ * and unused, as there (neighbours at distance 2 share the pixel's colour).
+ * Synchronises `stream` (the number of fixed-point passes depends on the
+ * frame) and returns when the frame is final.  Returns a
+ * defect_correct::DefectCorrectError value (0 Ok, 1 NullInput,
+ * 2 InvalidDimensions, 3 InvalidBitDepth, 5 ImageTooSmall below 5 x 5,
+ * 6 InternalError). */
+int imop_hip_defect_adaptive(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
+                             float threshold, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

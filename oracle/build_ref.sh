@@ -9,7 +9,8 @@
 # hosts; neither changes results without -ffast-math.
 #
 # A second library, libimop_ref_isp.so, holds the CPU white balance, colour
-# matrix, gamma and polynomial lens-shading stages.  Their module dispatch files (needed for the input
+# matrix, gamma, polynomial lens-shading and adaptive defect-correction
+# stages.  Their module dispatch files (needed for the input
 # validation they call) also reference the CUDA entry points, which a build
 # without CUDA does not define; compiling with hidden visibility and one
 # section per function and linking with --gc-sections drops the unused
@@ -59,6 +60,8 @@ ISP_SRCS=(
     "$REF/src/tone/dispatch.cpp"
     "$REF/src/lens_shading/polynomial.cpp"
     "$REF/src/lens_shading/dispatch.cpp"
+    "$REF/src/defect_correct/adaptive.cpp"
+    "$REF/src/defect_correct/dispatch.cpp"
     "$HERE/ref_isp_shim.cpp"
 )
 g++ -std=c++17 -O3 -mavx2 -mfma -fPIC -shared -w -fvisibility=hidden -ffunction-sections -fdata-sections \

@@ -15,6 +15,8 @@
  *   - gamma                    src/tone/gamma.cpp:10-34
  *   - polynomial lens shading  src/lens_shading/polynomial.cpp:11-52,
  *                              src/lens_shading/common.hpp:12-35
+ *   - adaptive defect correct. src/defect_correct/adaptive.cpp:8-61,
+ *                              src/defect_correct/common.hpp:12-16
  *   - helpers                  src/white_balance/common.hpp:13-47 (clamp_val,
  *                              safe_max_val, read_pixel, write_pixel; the ccm
  *                              and tone copies are the same)
@@ -35,8 +37,9 @@
  * reference sources by oracle/build_ref.sh) on random images with ordinary
  * and extreme parameters, against closed forms (unit gains, identity matrix
  * and gamma 1 leave an image unchanged; a permutation matrix permutes the
- * channels) and against the reference's lens-shading unit tests
- * (tests/unit/test_lens_shading.cpp:24-84).
+ * channels) and against the reference's lens-shading and defect-correction
+ * unit tests (tests/unit/test_lens_shading.cpp:24-84,
+ * tests/unit/test_defect_correct.cpp:8-72).
  */
 #include <math.h>
 #include <stddef.h>
@@ -151,4 +154,40 @@ void isp_lens_shading_poly(uint8_t *data, int w, int h, int pattern, int bd, con
             const size_t i = (size_t)y * (size_t)w + (size_t)x;
             wr(data, i, bd, clamp_val(cvt_x86(fmaf((float)rd(data, i, bd), gain, 0.5f)), mv));
         }
+}
+
+/* defect_correct::process_adaptive, in place on an unpacked Bayer frame (u8
+ * for bd <= 8, else u16): in raster order over [2, w-2) x [2, h-2), each
+ * pixel is compared with the median (element n/2 of the sorted values) of
+ * its same-colour neighbours at distance 2 that lie in that region, as they
+ * are at that moment (the ones above and to the left already corrected), and
+ * replaced by it when they differ by more than threshold * max.  Returns 5
+ * (ImageTooSmall) below 5 x 5. */
+int isp_defect_adaptive(uint8_t *data, int w, int h, int bd, float threshold) {
+    if (w < 5 || h < 5) return 5;
+    const int mv = bd <= 0 ? 255 : bd > 16 ? 65535 : (int)((1u << bd) - 1u);
+    /* std::max(0.05f, std::min(1.0f, threshold)) */
+    const float t_lo = (threshold < 1.0f) ? threshold : 1.0f;
+    const float t = (0.05f < t_lo) ? t_lo : 0.05f;
+    const int abs_thresh = (int)(t * (float)mv);
+    for (int y = 2; y < h - 2; ++y)
+        for (int x = 2; x < w - 2; ++x) {
+            const int center = rd(data, (size_t)y * w + x, bd);
+            int nb[8], n = 0;
+            for (int dy = -2; dy <= 2; dy += 2)
+                for (int dx = -2; dx <= 2; dx += 2) {
+                    if (dx == 0 && dy == 0) continue;
+                    const int nx = x + dx, ny = y + dy;
+                    if (nx >= 2 && nx < w - 2 && ny >= 2 && ny < h - 2) nb[n++] = rd(data, (size_t)ny * w + nx, bd);
+                }
+            if (n == 0) continue;
+            for (int i = 1; i < n; ++i) /* insertion sort: the sorted order std::sort gives */
+                for (int j = i; j > 0 && nb[j - 1] > nb[j]; --j) {
+                    const int tmp = nb[j]; nb[j] = nb[j - 1]; nb[j - 1] = tmp;
+                }
+            const int median = nb[n / 2];
+            const int diff = center > median ? center - median : median - center;
+            if (diff > abs_thresh) wr(data, (size_t)y * w + x, bd, median);
+        }
+    return 0;
 }

@@ -76,6 +76,7 @@ def oracle() -> ctypes.CDLL:
         lib.isp_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
         lib.isp_epilogue.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, _fp, ctypes.c_float]
         lib.isp_lens_shading_poly.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_float, ctypes.c_float]
+        lib.isp_defect_adaptive.argtypes = [_u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
         for fn in ("isp_wb_manual", "isp_ccm_linear", "isp_gamma", "isp_epilogue", "isp_lens_shading_poly"):
             getattr(lib, fn).restype = None
         lib.isp_gamma_clamp.argtypes = [ctypes.c_float]
@@ -117,6 +118,7 @@ def reference_isp() -> ctypes.CDLL:
         lib.ref_ccm_linear.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
         lib.ref_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
         lib.ref_lens_shading_poly.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_float, ctypes.c_float]
+        lib.ref_defect_adaptive.argtypes = [_u8p] + [ctypes.c_int] * 4 + [ctypes.c_float]
         _ref_isp = lib
     return _ref_isp
 
@@ -178,6 +180,23 @@ def lens_shading(bayer, pattern, bd, coef, center=(0.5, 0.5)):
 
 def ref_lens_shading(bayer, pattern, bd, coef, center=(0.5, 0.5)):
     return _lens_shading(reference_isp(), "ref_lens_shading_poly", bayer, pattern, bd, coef, center)
+
+
+def defect_adaptive(bayer, bd, threshold=0.3):
+    """defect_correct::process_adaptive (src/defect_correct/adaptive.cpp:8-61),
+    restated: returns (error, corrected copy)."""
+    out = np.ascontiguousarray(bayer).copy()
+    h, w = out.shape
+    e = oracle().isp_defect_adaptive(out.view(np.uint8).ctypes.data_as(_u8p), w, h, bd, ctypes.c_float(threshold))
+    return e, out
+
+
+def ref_defect_adaptive(bayer, bd, threshold=0.3, pattern=0):
+    out = np.ascontiguousarray(bayer).copy()
+    h, w = out.shape
+    e = reference_isp().ref_defect_adaptive(out.view(np.uint8).ctypes.data_as(_u8p), w, h, pattern, bd,
+                                            ctypes.c_float(threshold))
+    return e, out
 
 
 def ref_wb(rgb, bd, gains):

@@ -1,5 +1,6 @@
 // ref_isp_shim.cpp -- C entry points over the reference's own CPU white
-// balance, colour matrix, gamma and lens-shading stages, so tests can call the compiled
+// balance, colour matrix, gamma, lens-shading and adaptive defect-correction
+// stages, so tests can call the compiled
 // reference through ctypes.  TEST INFRASTRUCTURE ONLY: built by
 // oracle/build_ref.sh together with the reference sources under
 // /root/reference into oracle/_ref/libimop_ref_isp.so, used to pin
@@ -7,6 +8,7 @@
 #include <cstdint>
 
 #include "ccm/algorithms.hpp"
+#include "defect_correct/algorithms.hpp"
 #include "lens_shading/algorithms.hpp"
 #include "tone/algorithms.hpp"
 #include "white_balance/algorithms.hpp"
@@ -50,4 +52,12 @@ REF_EXPORT int ref_lens_shading_poly(uint8_t* data, int w, int h, int pattern, i
     p.center_x = cx;
     p.center_y = cy;
     return static_cast<int>(lens_shading::process_polynomial(data, w, h, static_cast<imop::BayerPattern>(pattern), bd, p));
+}
+
+// defect_correct::process_adaptive, src/defect_correct/adaptive.cpp:8-61 (in place)
+REF_EXPORT int ref_defect_adaptive(uint8_t* data, int w, int h, int pattern, int bd, float threshold) {
+    defect_correct::DefectCorrectParams p;
+    p.threshold = threshold;
+    return static_cast<int>(
+        defect_correct::process_adaptive(data, w, h, static_cast<imop::BayerPattern>(pattern), bd, p));
 }

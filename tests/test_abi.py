@@ -77,6 +77,9 @@ def test_no_device_behaves_like_reference_stub():
     ep = imop_amd.rgb_epilogue(gains=[2.0, 1.0, 1.5], gamma=2.2)
     assert lib.imop_hip_demosaic_isp(None, 0, None, 0, 8, 8, 0, 2, 8, 0, None, 0, ctypes.byref(ep), None) == 6
     assert lib.imop_hip_rgb_epilogue_apply(None, 0, 8, 8, 8, ctypes.byref(ep), None) == 6
+    ls = imop_amd.lens_shading_params([0.5, 0.0, 0.0] * 4)
+    assert lib.imop_hip_lens_shading_poly(None, 0, 8, 8, 0, 8, ctypes.byref(ls), None) == 6
+    assert lib.imop_hip_defect_adaptive(None, 0, 8, 8, 0, 8, ctypes.c_float(0.3), None) == 6
 
 
 @pytest.mark.parametrize("ctype,cname", [(imop_amd.FrameDesc, "imop_hip_frame_desc"),

@@ -1,9 +1,10 @@
 """The RGB epilogue oracle (oracle/isp_oracle.c: manual white balance, linear
-3x3 colour matrix, gamma) and the polynomial lens-shading oracle against the
-compiled reference stages (src/white_balance/manual.cpp, src/ccm/linear_3x3.cpp,
-src/tone/gamma.cpp, src/lens_shading/polynomial.cpp, built by
+3x3 colour matrix, gamma), the polynomial lens-shading oracle and the adaptive
+defect-correction oracle against the compiled reference stages
+(src/white_balance/manual.cpp, src/ccm/linear_3x3.cpp, src/tone/gamma.cpp,
+src/lens_shading/polynomial.cpp, src/defect_correct/adaptive.cpp, built by
 oracle/build_ref.sh), against closed forms and against the reference's
-lens-shading unit tests."""
+lens-shading and defect-correction unit tests."""
 from __future__ import annotations
 
 import numpy as np
@@ -163,3 +164,115 @@ def test_lens_shading_planted_mistake_is_detected():
     ref = O.ref_lens_shading(img, 0, bd, coef)
     assert not np.array_equal(naive, ref)
     np.testing.assert_array_equal(O.lens_shading(img, 0, bd, coef), ref)
+
+
+# ------------------------------------------------------------ defect correction
+
+def _defect_img(kind, bd, h, w, seed):
+    rng = np.random.default_rng(seed)
+    mx = (1 << bd) - 1
+    if kind == "noise":
+        img = rng.integers(0, mx + 1, size=(h, w))
+    elif kind == "sparse":   # smooth frame with hot and cold pixels
+        yy, xx = np.mgrid[0:h, 0:w]
+        img = (mx // 4 + (xx + 2 * yy) % max(1, mx // 8)) + rng.integers(0, max(1, mx // 64) + 1, size=(h, w))
+        img[rng.random((h, w)) < 0.03] = mx
+        img[rng.random((h, w)) < 0.01] = 0
+    else:                    # clusters of defects: chains of replacements
+        img = rng.integers(0, mx // 2 + 1, size=(h, w)) + (rng.random((h, w)) < 0.3) * (mx // 2)
+    return np.clip(img, 0, mx).astype(np.uint8 if bd <= 8 else np.uint16)
+
+
+@needs_ref
+@pytest.mark.parametrize("bd", [1, 8, 10, 12, 16])
+@pytest.mark.parametrize("kind", ["noise", "sparse", "clusters"])
+@pytest.mark.parametrize("thr", [0.3, 0.05, 0.12, 1.0, -2.0, float("nan")])
+def test_defect_adaptive_matches_reference(bd, kind, thr):
+    img = _defect_img(kind, bd, 41, 67, bd * 3 + len(kind))
+    e, got = O.defect_adaptive(img, bd, thr)
+    e_ref, want = O.ref_defect_adaptive(img, bd, thr)
+    assert e == e_ref == 0
+    np.testing.assert_array_equal(got, want)
+
+
+@needs_ref
+@pytest.mark.parametrize("h,w", [(5, 5), (5, 6), (6, 5), (6, 6), (7, 9), (5, 70), (70, 5), (8, 8)])
+def test_defect_adaptive_small_frames_match_reference(h, w):
+    for bd, kind in ((8, "noise"), (12, "clusters")):
+        img = _defect_img(kind, bd, h, w, h * 100 + w)
+        e, got = O.defect_adaptive(img, bd, 0.1)
+        e_ref, want = O.ref_defect_adaptive(img, bd, 0.1)
+        assert e == e_ref == 0
+        np.testing.assert_array_equal(got, want)
+    for h2, w2 in ((4, 9), (9, 4), (1, 1)):
+        z = np.zeros((h2, w2), np.uint8)
+        assert O.defect_adaptive(z, 8)[0] == O.ref_defect_adaptive(z, 8)[0] == 5   # ImageTooSmall
+
+
+def test_defect_reference_unit_tests():
+    """tests/unit/test_defect_correct.cpp:23-72 of the reference (adaptive cases)."""
+    # AdaptiveFixesHotPixel: flat 50 with a 255 at (4, 4), threshold 0.3
+    img = np.full((8, 8), 50, np.uint8)
+    img[4, 4] = 255
+    e, out = O.defect_adaptive(img, 8, 0.3)
+    assert e == 0 and out[4, 4] < 200
+    # AdaptivePreservesNormalPixels: flat 128 stays within 5
+    e, out = O.defect_adaptive(np.full((8, 8), 128, np.uint8), 8, 0.3)
+    assert e == 0 and np.abs(out.astype(int) - 128).max() <= 5
+    # OutputInRange: i % 256 ramp, default parameters (threshold 0.3)
+    e, out = O.defect_adaptive((np.arange(256) % 256).astype(np.uint8).reshape(16, 16), 8)
+    assert e == 0 and out.max() <= 255
+
+
+def test_defect_closed_forms():
+    """Exact answers fixed by the definition: a flat frame and threshold 1
+    leave every pixel; an isolated hot pixel in a flat frame becomes the flat
+    value; the border two pixels deep never changes."""
+    for bd in (8, 12):
+        mx = (1 << bd) - 1
+        dt = np.uint8 if bd <= 8 else np.uint16
+        flat = np.full((20, 30), mx // 3, dt)
+        np.testing.assert_array_equal(O.defect_adaptive(flat, bd, 0.05)[1], flat)
+        noise = _defect_img("noise", bd, 20, 30, 1)
+        np.testing.assert_array_equal(O.defect_adaptive(noise, bd, 1.0)[1], noise)   # |d| <= max never exceeds it
+        hot = flat.copy()
+        hot[9, 13] = mx
+        want = flat.copy()
+        np.testing.assert_array_equal(O.defect_adaptive(hot, bd, 0.3)[1], want)
+        out = O.defect_adaptive(noise, bd, 0.05)[1]
+        for sl in (np.s_[:2, :], np.s_[-2:, :], np.s_[:, :2], np.s_[:, -2:]):
+            np.testing.assert_array_equal(out[sl], noise[sl])
+
+
+def _defect_mistaken(img, bd, thr, sequential=True, rank=None):
+    """The stage with a planted mistake: all neighbours from the input
+    (sequential=False), or another element of the sorted neighbours (rank)."""
+    mx = (1 << bd) - 1
+    t = max(0.05, min(1.0, thr))
+    at = int(np.float32(t) * np.float32(mx))
+    h, w = img.shape
+    cur = img.astype(np.int64).copy()
+    src = cur if sequential else img.astype(np.int64)
+    for y in range(2, h - 2):
+        for x in range(2, w - 2):
+            nb = sorted(src[y + dy, x + dx] for dy in (-2, 0, 2) for dx in (-2, 0, 2)
+                        if (dy or dx) and 2 <= x + dx < w - 2 and 2 <= y + dy < h - 2)
+            if not nb:
+                continue
+            med = nb[len(nb) // 2 if rank is None else min(rank(len(nb)), len(nb) - 1)]
+            if abs(int(img[y, x]) - med) > at:
+                cur[y, x] = med
+    return cur
+
+
+@needs_ref
+def test_defect_planted_mistakes_are_detected():
+    """The non-sequential form (every neighbour from the input) and the lower
+    median differ from the reference; the oracle's raster order and element
+    n/2 match it."""
+    img = _defect_img("clusters", 8, 24, 40, 5)
+    want = O.ref_defect_adaptive(img, 8, 0.1)[1]
+    np.testing.assert_array_equal(O.defect_adaptive(img, 8, 0.1)[1], want)
+    np.testing.assert_array_equal(_defect_mistaken(img, 8, 0.1), want)
+    assert not np.array_equal(_defect_mistaken(img, 8, 0.1, sequential=False), want)
+    assert not np.array_equal(_defect_mistaken(img, 8, 0.1, rank=lambda n: (n - 1) // 2), want)
