@@ -38,22 +38,33 @@
 namespace imop_hip {
 namespace {
 
-constexpr int kTX = 128, kTY = 32;                      // tile (pixels)
-constexpr int kLW = kTX + 8, kLH = kTY + 4;             // LDS rows: x0-2 .. x0+TX+1 (+ pad), y0-2 .. y0+TY+1
-constexpr uint16_t kPad = 0xFFFF;                       // outside the interior: sorts after every value
-constexpr int kLocalCap = 1024;                         // > the longest chain inside a tile (TX/2 + TY + 2)
+constexpr int kTX = 128, kTY = 32;      // tile (pixels)
+constexpr int kLH = kTY + 4;            // LDS rows: y0-2 .. y0+TY+1
+constexpr int kQ = kTX / 2 + 2;         // pixel pairs per LDS row: x0-2 .. x0+TX+1
+constexpr int kQW = kQ + 2;             // row pitch (pairs)
+constexpr uint32_t kPad = 0xFFFFu;      // outside the interior: sorts after every value
+constexpr uint32_t kPad2 = 0xFFFFFFFFu;
+constexpr int kLocalCap = 1024;         // > the longest chain inside a tile (TX/2 + TY + 2)
 
 // Bits a tile records for the tiles its changes can reach.
 enum : uint32_t { kRight = 1, kLeft = 2, kBelowLeft = 4, kBelow = 8, kBelowRight = 16, kAny = 32 };
 
-__device__ __forceinline__ void ce(uint32_t& a, uint32_t& b) {
-    const uint32_t lo = min(a, b), hi = max(a, b);
+// Two pixels of one row, an even and an odd column (two colour planes, never
+// each other's neighbours), as the halves of a 32-bit word.
+typedef uint16_t px2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ px2 as2(uint32_t v) { return __builtin_bit_cast(px2, v); }
+__device__ __forceinline__ uint32_t as32(px2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+template <typename T>
+__device__ __forceinline__ void ce(T& a, T& b) {
+    const T lo = __builtin_elementwise_min(a, b), hi = __builtin_elementwise_max(a, b);
     a = lo;
     b = hi;
 }
 
 // Sorts four values in place (five compare-exchanges).
-__device__ __forceinline__ void sort4(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3) {
+template <typename T>
+__device__ __forceinline__ void sort4(T& a0, T& a1, T& a2, T& a3) {
     ce(a0, a1);
     ce(a2, a3);
     ce(a0, a2);
@@ -77,6 +88,51 @@ __device__ __forceinline__ uint32_t median_n2(uint32_t a0, uint32_t a1, uint32_t
     return min(a0, b0);
 }
 
+// F on a pair whose two pixels both have eight valid neighbours, in packed
+// 16-bit arithmetic: the median (k = 5), then med where |c - med| > t, else c.
+__device__ __forceinline__ uint32_t f_pair8(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t b0,
+                                            uint32_t b1, uint32_t b2, uint32_t b3, uint32_t c, px2 t) {
+    px2 A0 = as2(a0), A1 = as2(a1), A2 = as2(a2), A3 = as2(a3);
+    px2 B0 = as2(b0), B1 = as2(b1), B2 = as2(b2), B3 = as2(b3);
+    sort4(A0, A1, A2, A3);
+    sort4(B0, B1, B2, B3);
+    const px2 med = __builtin_elementwise_min(
+        __builtin_elementwise_min(__builtin_elementwise_max(A0, B3), __builtin_elementwise_max(A1, B2)),
+        __builtin_elementwise_min(__builtin_elementwise_max(A2, B1), __builtin_elementwise_max(A3, B0)));
+    const px2 C = as2(c);
+    const px2 diff = __builtin_elementwise_max(C, med) - __builtin_elementwise_min(C, med);
+    const px2 over = __builtin_elementwise_sub_sat(diff, t);                    // > 0 where diff > t
+    const uint32_t m = as32((px2){0, 0} - __builtin_elementwise_min(over, (px2){1, 1}));  // 0xFFFF there
+    return (as32(med) & m) | (c & ~m);
+}
+
+// F on a pair near the interior's edge, pixel by pixel (n from the position;
+// pixels outside the interior keep their value).
+__device__ __noinline__ uint32_t f_pair_edge(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t b0,
+                                             uint32_t b1, uint32_t b2, uint32_t b3, uint32_t c, uint32_t cur,
+                                             int x, int y, int w, int h, int t) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int sh = 16 * k, px = x + k;
+        uint32_t v = (cur >> sh) & 0xFFFFu;
+        if (px >= 2 && px < w - 2 && y >= 2 && y < h - 2) {
+            const int nx = 1 + (px - 2 >= 2) + (px + 2 < w - 2), ny = 1 + (y - 2 >= 2) + (y + 2 < h - 2);
+            const int n = nx * ny - 1;
+            if (n > 0) {
+                const uint32_t med = median_n2((a0 >> sh) & 0xFFFFu, (a1 >> sh) & 0xFFFFu, (a2 >> sh) & 0xFFFFu,
+                                               (a3 >> sh) & 0xFFFFu, (b0 >> sh) & 0xFFFFu, (b1 >> sh) & 0xFFFFu,
+                                               (b2 >> sh) & 0xFFFFu, (b3 >> sh) & 0xFFFFu, n);
+                const uint32_t cc = (c >> sh) & 0xFFFFu;
+                const uint32_t diff = cc > med ? cc - med : med - cc;
+                v = (int)diff > t ? med : cc;
+            }
+        }
+        out |= v << sh;
+    }
+    return out;
+}
+
 template <bool U16>
 __device__ __forceinline__ uint32_t ld_px(const uint8_t* p, size_t i) {
     return U16 ? (uint32_t)((const uint16_t*)p)[i] : (uint32_t)p[i];
@@ -88,12 +144,29 @@ __device__ __forceinline__ void st_px(uint8_t* p, size_t i, uint32_t v) {
     else p[i] = (uint8_t)v;
 }
 
+// The pair at (x, y) (x even) of a frame, kPad for pixels outside the
+// interior.  aligned: w even and the frame's base aligned to a pair, so the
+// pair is one load (the interior then holds whole pairs).
+template <bool U16>
+__device__ __forceinline__ uint32_t ld_pair(const uint8_t* p, int x, int y, int w, int h, bool aligned) {
+    if (y < 2 || y >= h - 2 || x < 2 || x >= w - 2) return kPad2;
+    const size_t i = (size_t)y * (size_t)w + (size_t)x;
+    if (aligned) {
+        if (U16) return *(const uint32_t*)((const uint16_t*)p + i);
+        const uint32_t v = *(const uint16_t*)(p + i);
+        return (v & 0xFFu) | ((v >> 8) << 16);
+    }
+    const uint32_t lo = ld_px<U16>(p, i);
+    const uint32_t hi = x + 1 < w - 2 ? ld_px<U16>(p, i + 1) : kPad;
+    return lo | (hi << 16);
+}
+
 struct DefectArgs {
     uint8_t* data;       // the frame, in place
     const uint8_t* orig; // its input values (scratch copy)
     uint8_t* flags;      // 2 x ntiles: reach bits of the previous / this pass
     int* ctl;            // [0..2] "pass changed something" ring, [3] error
-    int w, h, thresh, ntx, nty, pass;
+    int w, h, thresh, ntx, nty, pass, aligned;
 };
 
 __global__ void __launch_bounds__(256) k_defect_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
@@ -108,8 +181,9 @@ __global__ void __launch_bounds__(256) k_defect_copy(const uint4* __restrict__ s
 
 template <bool U16>
 __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
-    __shared__ uint16_t cur[kLH][kLW];
-    __shared__ uint16_t org[kLH][kLW];
+    __shared__ uint32_t cur[kLH][kQW];   // current values (pairs)
+    __shared__ uint32_t org[kLH][kQW];   // input values (pairs)
+    __shared__ uint16_t seen[kLH][kQW];  // local iteration in which the pair last changed (0: not yet)
     __shared__ uint32_t s_reach;
 
     const int tile = blockIdx.x;
@@ -143,72 +217,87 @@ __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
 
     const int x0 = tx * kTX, y0 = ty * kTY;
     const int w = a.w, h = a.h;
-    // Load tile + halo; positions outside the interior [2, w-2) x [2, h-2) hold kPad.
-    const uint8_t* cur_src = a.pass == 0 ? a.orig : a.data;
-    for (int i = tid; i < kLH * (kTX + 4); i += 256) {
-        const int r = i / (kTX + 4), c = i - r * (kTX + 4);
-        const int x = x0 - 2 + c, y = y0 - 2 + r;
-        uint32_t o = kPad, v = kPad;
-        if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2) {
-            const size_t gi = (size_t)y * (size_t)w + (size_t)x;
-            o = ld_px<U16>(a.orig, gi);
-            v = a.pass == 0 ? o : ld_px<U16>(cur_src, gi);
+    const bool aligned = a.aligned != 0;
+    // Load tile + halo (all loads issued before the LDS stores).
+    constexpr int kPairs = kLH * kQ, kPer = (kPairs + 255) / 256;
+    uint32_t vo[kPer], vc[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int k = tid + 256 * i;
+        const int r = k / kQ, q = k - r * kQ;
+        vo[i] = vc[i] = kPad2;
+        if (k < kPairs) {
+            vo[i] = ld_pair<U16>(a.orig, x0 - 2 + 2 * q, y0 - 2 + r, w, h, aligned);
+            if (a.pass > 0) vc[i] = ld_pair<U16>(a.data, x0 - 2 + 2 * q, y0 - 2 + r, w, h, aligned);
         }
-        org[r][c] = (uint16_t)o;
-        cur[r][c] = (uint16_t)v;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int k = tid + 256 * i;
+        if (k < kPairs) {
+            const int r = k / kQ, q = k - r * kQ;
+            org[r][q] = vo[i];
+            cur[r][q] = a.pass > 0 ? vc[i] : vo[i];
+            seen[r][q] = 0;
+        }
     }
     if (tid == 0) s_reach = 0;
     __syncthreads();
 
-    // This thread: columns x0 + 4g .. +3, rows y0 + 8 * wave + par + {0, 2, 4, 6}.
+    // This thread: pairs q = 2g+1, 2g+2 (pixels x0 + 4g .. +3) on rows
+    // y0 + 8 * wave + par + {0, 2, 4, 6}.
     const int lane = tid & 63, wave = tid >> 6;
     const int g = lane & 31, par = lane >> 5;
-    const int c0 = 4 * g + 2;  // LDS column of x0 + 4g
-    int nvalid[4][4];          // valid neighbour count per pixel, 0 = not in the interior
-    uint16_t start[4][4];      // values at load time
+    uint32_t kinds = 0;  // 2 bits per pair: 0 outside the interior, 1 edge, 2 eight neighbours
+    uint32_t start[4][2];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const int y = y0 + 8 * wave + par + 2 * s;
+        const int r = 8 * wave + par + 2 * s + 2, y = y0 + r - 2;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int x = x0 + 4 * g + j;
-            int n = 0;
-            if (x >= 2 && x < w - 2 && y >= 2 && y < h - 2) {
-                const int nx = 1 + (x - 2 >= 2) + (x + 2 < w - 2), ny = 1 + (y - 2 >= 2) + (y + 2 < h - 2);
-                n = nx * ny - 1;
-            }
-            nvalid[s][j] = n;
-            start[s][j] = cur[8 * wave + par + 2 * s + 2][c0 + j];
+        for (int k = 0; k < 2; ++k) {
+            const int q = 2 * g + 1 + k, x = x0 - 2 + 2 * q;
+            uint32_t kind = 0;
+            if (x >= 4 && x + 1 < w - 4 && y >= 4 && y < h - 4) kind = 2;
+            else if (y >= 2 && y < h - 2 && x + 1 >= 2 && x < w - 2) kind = 1;
+            kinds |= kind << (2 * (2 * s + k));
+            start[s][k] = cur[r][q];
         }
     }
+    const px2 t2 = (px2){(uint16_t)a.thresh, (uint16_t)a.thresh};
 
-    int iters = 0;
+    int it = 0;
     bool again;
     do {
+        ++it;
         bool changed = false;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int r = 8 * wave + par + 2 * s + 2;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = nvalid[s][j];
-                if (n == 0) continue;
-                const int c = c0 + j;
-                const uint32_t med =
-                    median_n2(cur[r - 2][c - 2], cur[r - 2][c], cur[r - 2][c + 2], cur[r][c - 2], org[r][c + 2],
-                              org[r + 2][c - 2], org[r + 2][c], org[r + 2][c + 2], n);
-                const uint32_t center = org[r][c];
-                const uint32_t diff = center > med ? center - med : med - center;
-                const uint32_t nv = (int)diff > a.thresh ? med : center;
-                if (nv != cur[r][c]) {
-                    cur[r][c] = (uint16_t)nv;
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t kind = (kinds >> (2 * (2 * s + k))) & 3u;
+                if (kind == 0) continue;
+                const int q = 2 * g + 1 + k;
+                if (it > 1) {  // unchanged inputs since the last evaluation: same value
+                    const int last = max(max((int)seen[r - 2][q - 1], (int)seen[r - 2][q]),
+                                         max((int)seen[r - 2][q + 1], (int)seen[r][q - 1]));
+                    if (last < it - 1) continue;
+                }
+                const uint32_t a0 = cur[r - 2][q - 1], a1 = cur[r - 2][q], a2 = cur[r - 2][q + 1], a3 = cur[r][q - 1];
+                const uint32_t b0 = org[r][q + 1], b1 = org[r + 2][q - 1], b2 = org[r + 2][q], b3 = org[r + 2][q + 1];
+                const uint32_t c = org[r][q], old = cur[r][q];
+                const uint32_t nv = kind == 2 ? f_pair8(a0, a1, a2, a3, b0, b1, b2, b3, c, t2)
+                                              : f_pair_edge(a0, a1, a2, a3, b0, b1, b2, b3, c, old, x0 - 2 + 2 * q,
+                                                            y0 + r - 2, w, h, a.thresh);
+                if (nv != old) {
+                    cur[r][q] = nv;
+                    seen[r][q] = (uint16_t)it;
                     changed = true;
                 }
             }
         }
-        ++iters;
         again = __syncthreads_or(changed) != 0;
-    } while (again && iters < kLocalCap);
+    } while (again && it < kLocalCap);
     if (again && tid == 0) a.ctl[3] = 1;  // cannot happen: the chain bound is below the cap
 
     uint32_t reach = 0;
@@ -217,18 +306,23 @@ __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
         const int r = 8 * wave + par + 2 * s + 2;
         const int y = y0 + r - 2;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int c = c0 + j;
-            const uint32_t v = cur[r][c];
-            if (nvalid[s][j] == 0 || v == start[s][j]) continue;
-            const int x = x0 + c - 2;
-            st_px<U16>(a.data, (size_t)y * (size_t)w + (size_t)x, v);
-            // successors: (x+2, y) and (x-2 .. x+2, y+2)
-            const bool right = x + 2 >= x0 + kTX, left = x - 2 < x0, down = y + 2 >= y0 + kTY;
-            reach |= kAny;
-            if (right) reach |= down ? (kRight | kBelowRight) : kRight;
-            if (left) reach |= down ? kBelowLeft : kLeft;
-            if (down) reach |= kBelow;
+        for (int k = 0; k < 2; ++k) {
+            const int q = 2 * g + 1 + k;
+            const uint32_t v = cur[r][q], v0 = start[s][k];
+            if (((kinds >> (2 * (2 * s + k))) & 3u) == 0 || v == v0) continue;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const uint32_t nvh = (v >> (16 * hh)) & 0xFFFFu;
+                if (nvh == ((v0 >> (16 * hh)) & 0xFFFFu)) continue;
+                const int x = x0 - 2 + 2 * q + hh;
+                st_px<U16>(a.data, (size_t)y * (size_t)w + (size_t)x, nvh);
+                // successors: (x+2, y) and (x-2 .. x+2, y+2)
+                const bool right = x + 2 >= x0 + kTX, left = x - 2 < x0, down = y + 2 >= y0 + kTY;
+                reach |= kAny;
+                if (right) reach |= down ? (kRight | kBelowRight) : kRight;
+                if (left) reach |= down ? kBelowLeft : kLeft;
+                if (down) reach |= kBelow;
+            }
         }
     }
     if (reach) atomicOr(&s_reach, reach);
@@ -258,6 +352,8 @@ int defect_tiles_y(int h) { return (h + kTY - 1) / kTY; }
 hipError_t launch_defect_pass(uint8_t* data, const uint8_t* orig, uint8_t* flags, int* ctl, int w, int h, int bd,
                               int thresh, int pass, hipStream_t st) {
     DefectArgs a;
+    // a pixel pair is one aligned load in both frames (the scratch copy is 256-byte aligned)
+    a.aligned = (w % 2 == 0) && ((uintptr_t)data % (bd > 8 ? 4u : 2u) == 0);
     a.data = data;
     a.orig = orig;
     a.flags = flags;

@@ -166,7 +166,9 @@ int defect_adaptive(uint8_t* data, int w, int h, int bd, float threshold, hipStr
     // the longest chain on a colour plane of pw x ph pixels is pw + 2 ph long.
     const long pw = (w - 4 + 1) / 2, ph = (h - 4 + 1) / 2;
     const long cap = pw + 2 * ph + 4;
-    int pass = 0, chunk = 2;
+    // Typical frames settle in two or three passes: check after four, then in
+    // growing chunks (passes after convergence return at once).
+    int pass = 0, chunk = 4;
     for (;;) {
         for (int i = 0; i < chunk; ++i, ++pass)
             if (launch_defect_pass(data, orig, flags, ctl, w, h, bd, thresh, pass, st) != hipSuccess)

@@ -104,6 +104,22 @@ def test_repeated_calls_and_stream():
             _check(img, 10, 0.1, stream=s)
 
 
+@pytest.mark.parametrize("bd,offset", [(8, 1), (12, 2)])
+def test_unaligned_buffer(bd, offset):
+    """A frame that starts off a pixel-pair boundary takes the per-pixel loads."""
+    h, w = 70, 300
+    img = _img("clusters", bd, h, w, 5)
+    raw = img.view(np.uint8).ravel()
+    buf = torch.zeros(raw.size + 8, dtype=torch.uint8, device=DEV)
+    buf[offset:offset + raw.size] = torch.from_numpy(raw.copy()).to(DEV)
+    d = buf[offset:offset + raw.size]
+    assert imop_amd.defect_correct(d, w, h, 0, bd, 0.1) == 0
+    out = d.cpu().numpy()
+    got = (out.view(np.uint16) if bd > 8 else out).reshape(h, w)
+    np.testing.assert_array_equal(got, O.defect_adaptive(img, bd, 0.1)[1])
+    assert not buf[:offset].any() and not buf[offset + raw.size:].any()
+
+
 def test_invalid_arguments():
     d = torch.zeros(64 * 64 * 2, dtype=torch.uint8, device=DEV)
     L = imop_amd.lib()
