@@ -41,7 +41,7 @@ namespace {
 constexpr int kTX = 128, kTY = 32;      // tile (pixels)
 constexpr int kLH = kTY + 4;            // LDS rows: y0-2 .. y0+TY+1
 constexpr int kQ = kTX / 2 + 2;         // pixel pairs per LDS row: x0-2 .. x0+TX+1
-constexpr int kQW = kQ + 2;             // row pitch (pairs)
+constexpr int kQW = kQ + 1;             // row pitch (pairs; odd: the two half-waves on different banks)
 constexpr uint32_t kPad = 0xFFFFu;      // outside the interior: sorts after every value
 constexpr uint32_t kPad2 = 0xFFFFFFFFu;
 constexpr int kLocalCap = 1024;         // > the longest chain inside a tile (TX/2 + TY + 2)
@@ -179,6 +179,149 @@ __global__ void __launch_bounds__(256) k_defect_copy(const uint4* __restrict__ s
     for (uint64_t i = n16 * 16u + t; i < nbytes; i += stride) dst_b[i] = src_b[i];
 }
 
+// One tile to its local fixed point, then its changed pixels to the frame.
+// EDGE: the tile holds pixels with fewer than eight valid neighbours (within
+// 4 of the frame's edge) or outside the interior; otherwise every pair takes
+// the packed path.
+template <bool U16, bool EDGE>
+__device__ __forceinline__ void defect_tile(const DefectArgs& a, int tile, int x0, int y0, uint32_t (*cur)[kQW],
+                                            uint32_t (*org)[kQW], uint16_t (*seen)[kQW], uint32_t* s_reach) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int w = a.w, h = a.h;
+    const bool aligned = a.aligned != 0;
+    // Load tile + halo: wave v takes rows v, v+4, ..., lane l the pair l (and
+    // lanes 0, 1 the pairs 64, 65); all loads issued before the LDS stores.
+    // Without EDGE the halo lies inside the interior: no bounds to check.
+    uint32_t vo[9][2], vc[9][2];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int r = wave + 4 * i, y = y0 - 2 + r;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int q = lane + 64 * e, x = x0 - 2 + 2 * q;
+            vo[i][e] = vc[i][e] = kPad2;
+            if (e == 0 || lane < kQ - 64) {
+                if (!EDGE && aligned) {
+                    const size_t gi = (size_t)y * (size_t)w + (size_t)x;
+                    if (U16) {
+                        vo[i][e] = *(const uint32_t*)((const uint16_t*)a.orig + gi);
+                        if (a.pass > 0) vc[i][e] = *(const uint32_t*)((const uint16_t*)a.data + gi);
+                    } else {
+                        const uint32_t o = *(const uint16_t*)(a.orig + gi);
+                        vo[i][e] = (o & 0xFFu) | ((o >> 8) << 16);
+                        if (a.pass > 0) {
+                            const uint32_t v = *(const uint16_t*)(a.data + gi);
+                            vc[i][e] = (v & 0xFFu) | ((v >> 8) << 16);
+                        }
+                    }
+                } else {
+                    vo[i][e] = ld_pair<U16>(a.orig, x, y, w, h, aligned);
+                    if (a.pass > 0) vc[i][e] = ld_pair<U16>(a.data, x, y, w, h, aligned);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int r = wave + 4 * i;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int q = lane + 64 * e;
+            if (e == 0 || lane < kQ - 64) {
+                org[r][q] = vo[i][e];
+                cur[r][q] = a.pass > 0 ? vc[i][e] : vo[i][e];
+                seen[r][q] = 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    // This thread: pairs q = 2g+1, 2g+2 (pixels x0 + 4g .. +3) on rows
+    // y0 + 8 * wave + par + {0, 2, 4, 6}.
+    const int g = lane & 31, par = lane >> 5;
+    uint32_t kinds = 0;  // EDGE: 2 bits per pair: 0 outside the interior, 1 edge, 2 eight neighbours
+    uint32_t start[4][2];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int r = 8 * wave + par + 2 * s + 2, y = y0 + r - 2;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int q = 2 * g + 1 + k, x = x0 - 2 + 2 * q;
+            if (EDGE) {
+                uint32_t kind = 0;
+                if (x >= 4 && x + 1 < w - 4 && y >= 4 && y < h - 4) kind = 2;
+                else if (y >= 2 && y < h - 2 && x + 1 >= 2 && x < w - 2) kind = 1;
+                kinds |= kind << (2 * (2 * s + k));
+            }
+            start[s][k] = cur[r][q];
+        }
+    }
+    const px2 t2 = (px2){(uint16_t)a.thresh, (uint16_t)a.thresh};
+
+    int it = 0;
+    bool again;
+    do {
+        ++it;
+        bool changed = false;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int r = 8 * wave + par + 2 * s + 2;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t kind = EDGE ? (kinds >> (2 * (2 * s + k))) & 3u : 2u;
+                if (EDGE && kind == 0) continue;
+                const int q = 2 * g + 1 + k;
+                if (it > 1) {  // unchanged inputs since the last evaluation: same value
+                    const int last = max(max((int)seen[r - 2][q - 1], (int)seen[r - 2][q]),
+                                         max((int)seen[r - 2][q + 1], (int)seen[r][q - 1]));
+                    if (last < it - 1) continue;
+                }
+                const uint32_t a0 = cur[r - 2][q - 1], a1 = cur[r - 2][q], a2 = cur[r - 2][q + 1], a3 = cur[r][q - 1];
+                const uint32_t b0 = org[r][q + 1], b1 = org[r + 2][q - 1], b2 = org[r + 2][q], b3 = org[r + 2][q + 1];
+                const uint32_t c = org[r][q], old = cur[r][q];
+                uint32_t nv;
+                if (!EDGE || kind == 2) nv = f_pair8(a0, a1, a2, a3, b0, b1, b2, b3, c, t2);
+                else nv = f_pair_edge(a0, a1, a2, a3, b0, b1, b2, b3, c, old, x0 - 2 + 2 * q, y0 + r - 2, w, h,
+                                      a.thresh);
+                if (nv != old) {
+                    cur[r][q] = nv;
+                    seen[r][q] = (uint16_t)it;
+                    changed = true;
+                }
+            }
+        }
+        again = __syncthreads_or(changed) != 0;
+    } while (again && it < kLocalCap);
+    if (again && tid == 0) a.ctl[3] = 1;  // cannot happen: the chain bound is below the cap
+
+    uint32_t reach = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int r = 8 * wave + par + 2 * s + 2;
+        const int y = y0 + r - 2;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int q = 2 * g + 1 + k;
+            const uint32_t v = cur[r][q], v0 = start[s][k];
+            if ((EDGE && ((kinds >> (2 * (2 * s + k))) & 3u) == 0) || v == v0) continue;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const uint32_t nvh = (v >> (16 * hh)) & 0xFFFFu;
+                if (nvh == ((v0 >> (16 * hh)) & 0xFFFFu)) continue;
+                const int x = x0 - 2 + 2 * q + hh;
+                st_px<U16>(a.data, (size_t)y * (size_t)w + (size_t)x, nvh);
+                // successors: (x+2, y) and (x-2 .. x+2, y+2)
+                const bool right = x + 2 >= x0 + kTX, left = x - 2 < x0, down = y + 2 >= y0 + kTY;
+                reach |= kAny;
+                if (right) reach |= down ? (kRight | kBelowRight) : kRight;
+                if (left) reach |= down ? kBelowLeft : kLeft;
+                if (down) reach |= kBelow;
+            }
+        }
+    }
+    if (reach) atomicOr(s_reach, reach);
+}
+
 template <bool U16>
 __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
     __shared__ uint32_t cur[kLH][kQW];   // current values (pairs)
@@ -214,118 +357,13 @@ __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
         if (tid == 0) flags_now[tile] = 0;
         return;
     }
-
-    const int x0 = tx * kTX, y0 = ty * kTY;
-    const int w = a.w, h = a.h;
-    const bool aligned = a.aligned != 0;
-    // Load tile + halo (all loads issued before the LDS stores).
-    constexpr int kPairs = kLH * kQ, kPer = (kPairs + 255) / 256;
-    uint32_t vo[kPer], vc[kPer];
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int k = tid + 256 * i;
-        const int r = k / kQ, q = k - r * kQ;
-        vo[i] = vc[i] = kPad2;
-        if (k < kPairs) {
-            vo[i] = ld_pair<U16>(a.orig, x0 - 2 + 2 * q, y0 - 2 + r, w, h, aligned);
-            if (a.pass > 0) vc[i] = ld_pair<U16>(a.data, x0 - 2 + 2 * q, y0 - 2 + r, w, h, aligned);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int k = tid + 256 * i;
-        if (k < kPairs) {
-            const int r = k / kQ, q = k - r * kQ;
-            org[r][q] = vo[i];
-            cur[r][q] = a.pass > 0 ? vc[i] : vo[i];
-            seen[r][q] = 0;
-        }
-    }
     if (tid == 0) s_reach = 0;
-    __syncthreads();
-
-    // This thread: pairs q = 2g+1, 2g+2 (pixels x0 + 4g .. +3) on rows
-    // y0 + 8 * wave + par + {0, 2, 4, 6}.
-    const int lane = tid & 63, wave = tid >> 6;
-    const int g = lane & 31, par = lane >> 5;
-    uint32_t kinds = 0;  // 2 bits per pair: 0 outside the interior, 1 edge, 2 eight neighbours
-    uint32_t start[4][2];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int r = 8 * wave + par + 2 * s + 2, y = y0 + r - 2;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int q = 2 * g + 1 + k, x = x0 - 2 + 2 * q;
-            uint32_t kind = 0;
-            if (x >= 4 && x + 1 < w - 4 && y >= 4 && y < h - 4) kind = 2;
-            else if (y >= 2 && y < h - 2 && x + 1 >= 2 && x < w - 2) kind = 1;
-            kinds |= kind << (2 * (2 * s + k));
-            start[s][k] = cur[r][q];
-        }
-    }
-    const px2 t2 = (px2){(uint16_t)a.thresh, (uint16_t)a.thresh};
-
-    int it = 0;
-    bool again;
-    do {
-        ++it;
-        bool changed = false;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int r = 8 * wave + par + 2 * s + 2;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const uint32_t kind = (kinds >> (2 * (2 * s + k))) & 3u;
-                if (kind == 0) continue;
-                const int q = 2 * g + 1 + k;
-                if (it > 1) {  // unchanged inputs since the last evaluation: same value
-                    const int last = max(max((int)seen[r - 2][q - 1], (int)seen[r - 2][q]),
-                                         max((int)seen[r - 2][q + 1], (int)seen[r][q - 1]));
-                    if (last < it - 1) continue;
-                }
-                const uint32_t a0 = cur[r - 2][q - 1], a1 = cur[r - 2][q], a2 = cur[r - 2][q + 1], a3 = cur[r][q - 1];
-                const uint32_t b0 = org[r][q + 1], b1 = org[r + 2][q - 1], b2 = org[r + 2][q], b3 = org[r + 2][q + 1];
-                const uint32_t c = org[r][q], old = cur[r][q];
-                const uint32_t nv = kind == 2 ? f_pair8(a0, a1, a2, a3, b0, b1, b2, b3, c, t2)
-                                              : f_pair_edge(a0, a1, a2, a3, b0, b1, b2, b3, c, old, x0 - 2 + 2 * q,
-                                                            y0 + r - 2, w, h, a.thresh);
-                if (nv != old) {
-                    cur[r][q] = nv;
-                    seen[r][q] = (uint16_t)it;
-                    changed = true;
-                }
-            }
-        }
-        again = __syncthreads_or(changed) != 0;
-    } while (again && it < kLocalCap);
-    if (again && tid == 0) a.ctl[3] = 1;  // cannot happen: the chain bound is below the cap
-
-    uint32_t reach = 0;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int r = 8 * wave + par + 2 * s + 2;
-        const int y = y0 + r - 2;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int q = 2 * g + 1 + k;
-            const uint32_t v = cur[r][q], v0 = start[s][k];
-            if (((kinds >> (2 * (2 * s + k))) & 3u) == 0 || v == v0) continue;
-#pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
-                const uint32_t nvh = (v >> (16 * hh)) & 0xFFFFu;
-                if (nvh == ((v0 >> (16 * hh)) & 0xFFFFu)) continue;
-                const int x = x0 - 2 + 2 * q + hh;
-                st_px<U16>(a.data, (size_t)y * (size_t)w + (size_t)x, nvh);
-                // successors: (x+2, y) and (x-2 .. x+2, y+2)
-                const bool right = x + 2 >= x0 + kTX, left = x - 2 < x0, down = y + 2 >= y0 + kTY;
-                reach |= kAny;
-                if (right) reach |= down ? (kRight | kBelowRight) : kRight;
-                if (left) reach |= down ? kBelowLeft : kLeft;
-                if (down) reach |= kBelow;
-            }
-        }
-    }
-    if (reach) atomicOr(&s_reach, reach);
+    const int x0 = tx * kTX, y0 = ty * kTY;
+    // every pixel of the tile with eight valid neighbours: [4, w-4) x [4, h-4)
+    if (x0 >= 4 && x0 + kTX <= a.w - 4 && y0 >= 4 && y0 + kTY <= a.h - 4)
+        defect_tile<U16, false>(a, tile, x0, y0, cur, org, seen, &s_reach);
+    else
+        defect_tile<U16, true>(a, tile, x0, y0, cur, org, seen, &s_reach);
     __syncthreads();
     if (tid == 0) {
         flags_now[tile] = (uint8_t)s_reach;
