@@ -20,7 +20,9 @@
 //                  halo into LDS (current values from the frame, or from O in
 //                  pass 0; originals from O) and iterates F over its pixels in
 //                  place until an iteration changes nothing, the halo held
-//                  fixed; then writes its changed pixels to the frame.  A tile
+//                  fixed; then writes its changed pixels to the frame (a tile
+//                  run again in a later pass starts from the pairs next to its
+//                  halo: the rest were consistent when it last ran).  A tile
 //                  records which neighbouring tiles its changes can reach (the
 //                  right one through (x+2, y), the left and the three below
 //                  through row y+2); pass k+1 re-runs only the tiles reached,
@@ -271,6 +273,11 @@ __device__ __forceinline__ void defect_tile(const DefectArgs& a, int tile, int x
                 const uint32_t kind = EDGE ? (kinds >> (2 * (2 * s + k))) & 3u : 2u;
                 if (EDGE && kind == 0) continue;
                 const int q = 2 * g + 1 + k;
+                // A tile re-run in a later pass sat at its fixed point with the halo it
+                // loaded then, and only it writes its pixels: only pairs with an input
+                // in the halo (top two rows; first and last pair columns, through
+                // (x-2, y) and (x+2, y-2)) can change first.
+                if (it == 1 && a.pass > 0 && r >= 4 && q != 1 && q != kQ - 2) continue;
                 if (it > 1) {  // unchanged inputs since the last evaluation: same value
                     const int last = max(max((int)seen[r - 2][q - 1], (int)seen[r - 2][q]),
                                          max((int)seen[r - 2][q + 1], (int)seen[r][q - 1]));
