@@ -812,6 +812,7 @@ def main():
         return
     if args.only == "defect":
         out = bench_defect(args.steps, args.warmup, False)
+        calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
 
