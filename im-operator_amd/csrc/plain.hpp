@@ -22,6 +22,6 @@ hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q
 
 // PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
 size_t prism_workspace_bytes(int w, int h);
-hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st);
+hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st, bool rcd = false);
 
 }  // namespace imop_hip

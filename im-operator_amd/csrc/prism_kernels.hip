@@ -1,5 +1,8 @@
 // prism_kernels.hip -- PRISM edge-directed demosaic (src/algorithms/prism.cpp:
-// 25-316) on gfx950 in two LDS-tiled passes.
+// 25-316) and RCD (src/algorithms/rcd.cpp:7-248) on gfx950 in two LDS-tiled
+// passes.  RCD's stages are PRISM's S0-S6 without the Laplacian term of the
+// green estimate (prism.cpp:132-136), and it has no colour-difference median
+// (S7-S9): pass B sends S6's planes straight to the border fill.
 //
 //   k_prism_a  raw -> CFA (decode + black level, S0), VH map (S1), low-pass
 //              (S2), PQ map (S4) and green at R/B (S3); writes four FP32
@@ -131,8 +134,9 @@ __device__ __forceinline__ float stage_pq(Cf C) {
     return ps / (ps + qs);
 }
 
-// S3: green at R/B sites (prism.cpp:114-163), inside [4, h-4) x [4, w-4)
-template <class Cf, class Vf, class Lf>
+// S3: green at R/B sites (prism.cpp:114-163; rcd.cpp:88-127 without the
+// Laplacian), inside [4, h-4) x [4, w-4)
+template <bool RCD, class Cf, class Vf, class Lf>
 __device__ __forceinline__ float stage_s3(Cf C, Vf V, Lf L) {
     float vc = V(0, 0);
     float vn = 0.25f * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1));
@@ -149,10 +153,12 @@ __device__ __forceinline__ float stage_s3(Cf C, Vf V, Lf L) {
     float eE = C(1, 0) * (1.0f + (l0 - lE) / (kEps + l0 + lE));
     float vEst = fmaf(gS, eN, gN * eS) / (gN + gS);
     float hEst = fmaf(gW, eE, gE * eW) / (gE + gW);
-    float lapV = (2.0f * c0 - C(0, -2) - C(0, 2)) * 0.25f;
-    float lapH = (2.0f * c0 - C(-2, 0) - C(2, 0)) * 0.25f;
-    vEst += lapV;
-    hEst += lapH;
+    if (!RCD) {
+        float lapV = (2.0f * c0 - C(0, -2) - C(0, 2)) * 0.25f;
+        float lapH = (2.0f * c0 - C(-2, 0) - C(2, 0)) * 0.25f;
+        vEst += lapV;
+        hEst += lapH;
+    }
     return clamp01(fmaf(disc, hEst, (1.0f - disc) * vEst));
 }
 
@@ -225,6 +231,7 @@ struct Tile {
 
 constexpr int AC = 5;  // CFA apron of pass A
 
+template <bool RCD>
 __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
     __shared__ float s_cfa[Tile<AC>::H][Tile<AC>::W];  // tile + 5
     __shared__ float s_vh[Tile<1>::H][Tile<1>::W];     // tile + 1
@@ -284,7 +291,7 @@ __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
         auto C = [&](int dx, int dy) { return s_cfa[cj + dy][ci + dx]; };
         const bool in4 = inside(x, y, w, h, 4);
         P.pq[o] = in4 ? stage_pq(C) : 0.5f;
-        P.c1[o] = in4 ? stage_s3(C, [&](int dx, int dy) { return s_vh[j + 1 + dy][i + 1 + dx]; },
+        P.c1[o] = in4 ? stage_s3<RCD>(C, [&](int dx, int dy) { return s_vh[j + 1 + dy][i + 1 + dx]; },
                                  [&](int dx, int dy) { return s_lp[j + 2 + dy][i + 2 + dx]; })
                       : 0.0f;
     }
@@ -295,6 +302,7 @@ __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
 
 // pass B reads CFA with a 7-pixel apron, green 6, PQ 5, VH 2; chroma planes
 // cover the tile + 4
+template <bool RCD>
 __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
     __shared__ float s_cfa[Tile<7>::H][Tile<7>::W];
     __shared__ float s_g[Tile<6>::H][Tile<6>::W];
@@ -367,10 +375,11 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
         s_b[rj][ri] = bv;
     }
     __syncthreads();
-    // S7-S9: medians of the colour differences over the tile; R into s_pq,
-    // B into s_vh (both dead now), tile coordinates
+    // S7-S9 (PRISM): medians of the colour differences over the tile; R into
+    // s_pq, B into s_vh (both dead now), tile coordinates
     float* s_r9 = &s_pq[0][0];
     float* s_b9 = &s_vh[0][0];
+    if (!RCD)
     for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
         const int j = k / TW, i = k - j * TW;
         const int x = x0 + i, y = y0 + j;
@@ -409,9 +418,12 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
         else if (y >= h - 4) { sy = h - 5; sx = x; }
         else { sy = y; sx = x < 4 ? 4 : (x >= w - 4 ? w - 5 : x); }
         const int q = (sy - y0) * TW + (sx - x0);
-        int r = (int)fmaf(s_r9[q], scale, 0.5f);
+        // RCD: S6's chroma planes (apron 4); PRISM: the medians' output
+        const float rv = RCD ? s_r[sy - y0 + 4][sx - x0 + 4] : s_r9[q];
+        const float bv = RCD ? s_b[sy - y0 + 4][sx - x0 + 4] : s_b9[q];
+        int r = (int)fmaf(rv, scale, 0.5f);
         int g = (int)fmaf(s_g[sy - y0 + 6][sx - x0 + 6], scale, 0.5f);
-        int b = (int)fmaf(s_b9[q], scale, 0.5f);
+        int b = (int)fmaf(bv, scale, 0.5f);
         r = iclamp(r, 0, f.maxv); g = iclamp(g, 0, f.maxv); b = iclamp(b, 0, f.maxv);
         put_px(f, x, y, r, g, b);
     }
@@ -421,10 +433,15 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
 
 size_t prism_workspace_bytes(int w, int h) { return 4u * plane_stride(w, h) * sizeof(float); }
 
-hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st) {
+hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st, bool rcd) {
     const dim3 grd(((f.w + TW - 1) / TW) * ((f.h + TH - 1) / TH));
-    hipLaunchKernelGGL(k_prism_a, grd, dim3(256), 0, st, f, ws);
-    hipLaunchKernelGGL(k_prism_b, grd, dim3(256), 0, st, f, ws);
+    if (rcd) {
+        hipLaunchKernelGGL(k_prism_a<true>, grd, dim3(256), 0, st, f, ws);
+        hipLaunchKernelGGL(k_prism_b<true>, grd, dim3(256), 0, st, f, ws);
+    } else {
+        hipLaunchKernelGGL(k_prism_a<false>, grd, dim3(256), 0, st, f, ws);
+        hipLaunchKernelGGL(k_prism_b<false>, grd, dim3(256), 0, st, f, ws);
+    }
     return hipGetLastError();
 }
 

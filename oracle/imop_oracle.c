@@ -18,6 +18,9 @@
  *   - MG ("Malvar-He-Cutler")           src/algorithms/mg.cpp:8-217,
  *                                       src/algorithms/avx2.cpp:422-687
  *   - PRISM (edge-directed)             src/algorithms/prism.cpp:25-316
+ *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
+ *                                       stages S0-S6 without the Laplacian
+ *                                       term, then borders and output)
  *   - border replication                src/algorithms/common.hpp:47-129
  *
  * The reference CPU library is built with -mavx2 (CMakeLists.txt:222-229), so
@@ -428,6 +431,9 @@ static void mg_float(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int b
  * contraction cannot change a result, are written plainly.) */
 static int g_prism_fma = 1;
 void orc_set_prism_fma(int on) { g_prism_fma = on; }
+/* A planted mistake for the tests: RCD with PRISM's Laplacian term kept. */
+static int g_rcd_laplacian = 0;
+void orc_set_rcd_laplacian(int on) { g_rcd_laplacian = on; }
 static inline float mad(float a, float b, float c) { return g_prism_fma ? fmaf(a, b, c) : a * b + c; }
 
 /* std::min / std::max semantics (first argument wins ties). */
@@ -450,8 +456,12 @@ static float median9(float *v) {
 
 /* process_prism (prism.cpp:25-316), stage by stage.  Planes: cfa, the three
  * colour planes, VH and PQ direction maps, the R/B low-pass, and the two
- * colour-difference planes with their median-filtered copies. */
-static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) {
+ * colour-difference planes with their median-filtered copies.
+ * rcd: process_rcd (rcd.cpp:7-248), whose stages are the same statements
+ * (rcd.cpp:43-224 = prism.cpp:68-264) except that its green estimate has no
+ * Laplacian term (prism.cpp:132-136) and it has no colour-difference median
+ * (prism.cpp:266-300): S6's planes go straight to the border fill. */
+static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, int rcd) {
     const size_t n = (size_t)w * h;
     const int mx = orc_max_val(bd);
     const float scale = (float)mx;
@@ -529,10 +539,12 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) 
             float eE = cfa[i + 1] * (1.0f + (l0 - lE) / (eps + l0 + lE));
             float vEst = mad(gS, eN, gN * eS) / (gN + gS);
             float hEst = mad(gW, eE, gE * eW) / (gE + gW);
-            float lapV = (2.0f * c0 - cfa[i - 2 * W] - cfa[i + 2 * W]) * 0.25f;
-            float lapH = (2.0f * c0 - cfa[i - 2] - cfa[i + 2]) * 0.25f;
-            vEst += lapV;
-            hEst += lapH;
+            if (!rcd || g_rcd_laplacian) {
+                float lapV = (2.0f * c0 - cfa[i - 2 * W] - cfa[i + 2 * W]) * 0.25f;
+                float lapH = (2.0f * c0 - cfa[i - 2] - cfa[i + 2]) * 0.25f;
+                vEst += lapV;
+                hEst += lapH;
+            }
             G[i] = clamp01(mad(disc, hEst, (1.0f - disc) * vEst));
         }
     }
@@ -609,10 +621,12 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) 
     }
 
     /* S7: colour differences (:266-269) */
+    if (!rcd)
     for (size_t i = 0; i < n; ++i) { L[i] = C[0][i] - G[i]; M[i] = C[2][i] - G[i]; }
 
     /* S8: 3x3 median of both differences into separate planes; the outer
      * ring stays 0 and replaces L/M wholesale (:273-292) */
+    if (!rcd)
     for (int y = 1; y < h - 1; ++y)
         for (int x = 1; x < w - 1; ++x) {
             float a[9], b[9];
@@ -628,6 +642,7 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) 
         }
 
     /* S9: reconstruct R and B from the filtered differences (:294-300) */
+    if (!rcd)
     for (int y = 1; y < h - 1; ++y)
         for (int x = 1; x < w - 1; ++x) {
             size_t i = (size_t)y * w + x;
@@ -635,7 +650,7 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) 
             C[2][i] = clamp01(G[i] + Mm[i]);
         }
 
-    /* S10 / S11: plane borders 4, scale, round, clamp (:302-313) */
+    /* S10 / S11: plane borders 4, scale, round, clamp (:302-313; rcd.cpp:226-245) */
     plane_borders_f(C[0], C[1], C[2], w, h, 4);
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
@@ -652,11 +667,11 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) 
     free(L); free(M); free(Lm); free(Mm);
 }
 
-/* demosaic_cpu (dispatch.cpp:115-128) restricted to HQLI / MG / PRISM:
+/* demosaic_cpu (dispatch.cpp:115-128) restricted to HQLI / MG / PRISM / RCD:
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
- * (hqli.cpp:11, mg.cpp:12, prism.cpp:29).  Other algorithms return
- * InternalError: they are not restated. */
+ * (hqli.cpp:11, mg.cpp:12, prism.cpp:29, rcd.cpp:11).  Other algorithms
+ * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
     if (w <= 0 || h <= 0) return E_DIMS;
@@ -664,7 +679,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (pattern < 0 || pattern > 3) return E_PATTERN;
     int min_dim;
     if (algo == A_HQLI || algo == A_MG) min_dim = 6;
-    else if (algo == A_PRISM) min_dim = 10;
+    else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
     const phase_t ph = phase_of(pattern);
@@ -675,7 +690,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);
         else mg_float(P, rgb, w, h, ph, bd);
-    } else prism(P, rgb, w, h, ph, bd);
+    } else prism(P, rgb, w, h, ph, bd, algo == A_RCD);
     free(P);
     return E_OK;
 }

@@ -127,7 +127,7 @@ def test_hqli_flat_field_kat():
     assert tuple(rgb[5, 4]) == (200, 120, 80)          # Gb
 
 
-@pytest.mark.parametrize("algo", [O.HQLI, O.MG, O.PRISM])
+@pytest.mark.parametrize("algo", [O.HQLI, O.MG, O.PRISM, O.RCD])
 @pytest.mark.parametrize("bd,packed", [(8, False), (12, True), (10, True), (16, False)])
 def test_constant_field(algo, bd, packed):
     # a constant raw frame demosaics to the same constant everywhere the
@@ -141,7 +141,7 @@ def test_constant_field(algo, bd, packed):
     # margins: HQLI's stencil is exact from row/column 2; MG's colour
     # differences read the sentinel G of row/column 1; PRISM's chain of
     # stages reaches about 10 px in from its zero-initialised rim.
-    m = {O.HQLI: 2, O.MG: 3, O.PRISM: 10}[algo]
+    m = {O.HQLI: 2, O.MG: 3, O.PRISM: 10, O.RCD: 10}[algo]
     core = rgb[m:h - m, m:w - m]
     assert np.all(core == v)
     if algo == O.HQLI:
@@ -188,11 +188,59 @@ def test_planted_prism_fma_is_pinned():
 
 
 @pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (10, True), (12, True), (14, False), (16, False)])
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+def test_rcd_matches_compiled_reference(bd, packed, pattern):
+    """RCD (rcd.cpp:7-248) against the compiled reference on scenes and noise,
+    with and without black level."""
+    for k, (h, w) in enumerate(((10, 10), (37, 61), (48, 130))):
+        vals = scene.make_scene(h, w, bd, pattern, seed=bd * 13 + pattern + k)
+        if k == 1:
+            vals = np.random.default_rng(k + bd).integers(0, 1 << bd, size=(h, w))
+        raw = scene.raw_bytes(vals, bd, packed)
+        bl = [float(16 * k)] * 4 if k else None
+        e1, o1 = O.demosaic(raw, w, h, pattern, O.RCD, bd, packed, bl)
+        e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.RCD, bd, packed, bl)
+        assert e1 == e2 == 0
+        np.testing.assert_array_equal(o1, o2)
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+def test_planted_rcd_mistakes_are_detected():
+    """RCD with PRISM's Laplacian term kept, and RCD without GCC's FMA
+    contraction, both differ from the compiled reference: the oracle's
+    statements are pinned, not assumed."""
+    lib = O.oracle()
+    lib.orc_set_prism_fma.argtypes = [ctypes.c_int]
+    lib.orc_set_rcd_laplacian.argtypes = [ctypes.c_int]
+    w, h, bd = 96, 64, 12
+    vals = scene.make_scene(h, w, bd, 0, seed=3)
+    raw = scene.raw_bytes(vals, bd, True)
+    want = O.ref_demosaic(raw, w, h, 0, O.RCD, bd, True)[1]
+    np.testing.assert_array_equal(O.demosaic(raw, w, h, 0, O.RCD, bd, True)[1], want)
+    lib.orc_set_rcd_laplacian(1)
+    try:
+        assert not np.array_equal(O.demosaic(raw, w, h, 0, O.RCD, bd, True)[1], want)
+    finally:
+        lib.orc_set_rcd_laplacian(0)
+    # the contraction sites move a 16-bit output by one step here and there
+    vals = np.random.default_rng(0).integers(0, 65536, size=(96, 128))
+    raw = scene.raw_bytes(vals, 16, False)
+    want = O.ref_demosaic(raw, 128, 96, 0, O.RCD, 16, False)[1]
+    np.testing.assert_array_equal(O.demosaic(raw, 128, 96, 0, O.RCD, 16, False)[1], want)
+    lib.orc_set_prism_fma(0)
+    try:
+        assert not np.array_equal(O.demosaic(raw, 128, 96, 0, O.RCD, 16, False)[1], want)
+    finally:
+        lib.orc_set_prism_fma(1)
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
 def test_against_compiled_reference_random():
     rng = np.random.default_rng(77)
-    for t in range(60):
-        algo = (O.HQLI, O.MG, O.PRISM)[t % 3]
-        md = 10 if algo == O.PRISM else 6
+    for t in range(80):
+        algo = (O.HQLI, O.MG, O.PRISM, O.RCD)[t % 4]
+        md = 10 if algo in (O.PRISM, O.RCD) else 6
         w = int(rng.integers(md, 70))
         h = int(rng.integers(md, 50))
         bd, packed = [(8, False), (12, True), (10, True), (14, True), (16, False), (11, False)][t % 6]
@@ -210,7 +258,8 @@ def test_against_compiled_reference_random():
 def test_error_codes_match_reference():
     raw = np.zeros(64 * 64 * 2, dtype=np.uint8)
     for (w, h, bd, pat, algo) in [(0, 8, 8, 0, O.HQLI), (8, -1, 8, 0, O.MG), (8, 8, 17, 0, O.HQLI),
-                                  (8, 8, 8, 9, O.MG), (5, 8, 8, 0, O.HQLI), (9, 12, 8, 0, O.PRISM)]:
+                                  (8, 8, 8, 9, O.MG), (5, 8, 8, 0, O.HQLI), (9, 12, 8, 0, O.PRISM),
+                                  (12, 9, 8, 0, O.RCD)]:
         lib = O.oracle()
         out1 = np.zeros(64 * 64 * 6, dtype=np.uint8)
         out2 = np.zeros(64 * 64 * 6, dtype=np.uint8)
