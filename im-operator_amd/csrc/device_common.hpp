@@ -39,6 +39,7 @@ struct FrameParams {
     int32_t pattern;      // BayerPattern, 0..3
     int32_t out16;        // RGB written as u16 (bd > 8, or on request for 8-bit frames)
     float bl_q[4];        // offset by (y & 1) * 2 + (x & 1), already max(0, .)
+    int32_t byte_path;    // 8-bit samples demosaiced from an unpacked buffer (the reference's 8-bit routines)
 };
 
 // Strips of the register-resident kernel.  A wave computes 248 columns

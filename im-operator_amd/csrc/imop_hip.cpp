@@ -45,7 +45,7 @@ int check_args(const void* in, const void* out, int w, int h, int pattern, int a
     if (bd < 1 || bd > 16) return kBitDepth;
     if (pattern < 0 || pattern > 3) return kPattern;
     if (w < min_dim_of(algo) || h < min_dim_of(algo)) return kSmall;
-    if (algo != kHqli && algo != kMg && algo != kPrism && algo != kRcd) return kInternal;
+    if (algo != kSuperFast && algo != kHqli && algo != kMg && algo != kPrism && algo != kRcd) return kInternal;
     // The reference reads packed streams below 8 bits as one byte per pixel
     // (pixel::get_raw), past the end of the stream; leave those to it.
     if (packed && bd < 8) return kInternal;
@@ -93,6 +93,7 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     // With black level the demosaic runs on the unpacked buffer.
     const bool eff_packed = packed && !f.bl_on;
     const bool byte_path = bd <= 8 && !eff_packed;
+    f.byte_path = byte_path ? 1 : 0;
     if (algo == kHqli) {
         *mode = MODE_HQLI;
         f.clamp_hi = byte_path ? 255 : f.maxv;
@@ -146,11 +147,13 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
     // max + 1 <= 4096, bit depths up to 12); otherwise the epilogue runs as
     // its own pass
     const uint32_t lds_entries = f.out16 ? (uint32_t)f.maxv + 1u : 256u;
-    if (with_epi && algo != kPrism && algo != kRcd && fast_path_ok(f) && (!(epi->flags & EPI_GAMMA) || lds_entries <= 4096u)) {
+    if (with_epi && (algo == kHqli || algo == kMg) && fast_path_ok(f) && (!(epi->flags & EPI_GAMMA) || lds_entries <= 4096u)) {
         he = launch_fast(f, mode, st, epi);
         return he == hipSuccess ? kOk : kInternal;
     }
-    if (algo == kPrism || algo == kRcd) {
+    if (algo == kSuperFast) {
+        he = launch_super_fast(f, st);
+    } else if (algo == kPrism || algo == kRcd) {
         // the PRISM / RCD passes index their planes with 32-bit offsets
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;
         void* ws = prism_ws;

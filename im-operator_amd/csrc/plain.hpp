@@ -20,6 +20,9 @@ void get_launch_shape(int* rows, int* wpb, int* xcd_remap);
 // In-place black level on an unpacked frame; q = offsets by (y & 1) * 2 + (x & 1).
 hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q[4], hipStream_t st);
 
+// SUPER_FAST for one frame (super_fast_kernels.hip).
+hipError_t launch_super_fast(const FrameParams& f, hipStream_t st);
+
 // PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
 size_t prism_workspace_bytes(int w, int h);
 hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st, bool rcd = false);

@@ -18,6 +18,8 @@
  *   - MG ("Malvar-He-Cutler")           src/algorithms/mg.cpp:8-217,
  *                                       src/algorithms/avx2.cpp:422-687
  *   - PRISM (edge-directed)             src/algorithms/prism.cpp:25-316
+ *   - SUPER_FAST (nearest / 2-4 average) src/algorithms/super_fast.cpp:9-43,
+ *                                       avx2.cpp:99-253
  *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
  *                                       stages S0-S6 without the Laplacian
  *                                       term, then borders and output)
@@ -249,6 +251,64 @@ DEFINE_PLANE_BORDERS(plane_borders_f, float)
 
 static int g_mg8_scalar = 0;
 void orc_set_mg8_scalar(int on) { g_mg8_scalar = on; }
+
+/* ------------------------------------------------------------ SUPER_FAST */
+/* process_super_fast (super_fast.cpp:9-43) on an AVX2 host, which is where
+ * the reference takes process_super_fast_optimized (avx2.cpp:209-253):
+ *   R/B site: own value, green = (l + r + u + d + 2) >> 2, the other chroma
+ *             from the diagonal (x +- 1 toward the pair, y +- 1), clamped;
+ *   G site:   own value, the row's chroma = (l + r + 1) >> 1, the column's
+ *             chroma = (u + d + 1) >> 1;
+ * every neighbour clamped into the frame.  8-bit unpacked frames take
+ * super_fast_avx2_8bit (avx2.cpp:99-207): 32-pixel vectors whose "left"
+ * operand is loaded from x - 1 except in a row's first vector (loaded from
+ * 0: pixels 1..31 see themselves as their left neighbour), and whose "right"
+ * operand comes from x + 1 except in a vector starting at w - 32 (when w is
+ * a multiple of 32: pixels w-32..w-1 see themselves); the scalar tail is
+ * exact.  g_sf_scalar = 1
+ * gives the scalar core (interp_core.hpp:73-118) instead: the exact
+ * neighbours everywhere. */
+static int g_sf_scalar = 0;
+void orc_set_super_fast_scalar(int on) { g_sf_scalar = on; }
+
+static void super_fast(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, int byte_path) {
+    const int mx = orc_max_val(bd);
+    const int quirks = byte_path && !g_sf_scalar;
+    const int full = (w / 32) * 32;  /* pixels covered by 32-pixel vectors */
+    for (int y = 0; y < h; ++y) {
+        const int yu = y > 0 ? y - 1 : 0, yd = y < h - 1 ? y + 1 : h - 1;
+        for (int x = 0; x < w; ++x) {
+            const int xl = x > 0 ? x - 1 : 0, xr = x < w - 1 ? x + 1 : w - 1;
+            int l = P[(size_t)y * w + xl], r = P[(size_t)y * w + xr];
+            if (quirks && x < full) {
+                if (x < 32) l = P[(size_t)y * w + x];
+                if ((x & ~31) == w - 32) r = P[(size_t)y * w + x];
+            }
+            const int c = P[(size_t)y * w + x], u = P[(size_t)yu * w + x], d = P[(size_t)yd * w + x];
+            const int s = site(&ph, y, x);
+            int R, G, B;
+            if (s != 1) {
+                int nx = (x & 1) ? x - 1 : x + 1, ny = (y & 1) ? y - 1 : y + 1;
+                nx = nx < 0 ? 0 : (nx >= w ? w - 1 : nx);
+                ny = ny < 0 ? 0 : (ny >= h ? h - 1 : ny);
+                const int diag = P[(size_t)ny * w + nx];
+                G = (l + r + u + d + 2) >> 2;
+                R = s == 0 ? c : diag;
+                B = s == 0 ? diag : c;
+            } else {
+                G = c;
+                const int hz = (l + r + 1) >> 1, vt = (u + d + 1) >> 1;
+                const int gr_row = (y & 1) == ph.rr;
+                R = gr_row ? hz : vt;
+                B = gr_row ? vt : hz;
+            }
+            if (!byte_path) { /* set_rgb_clamped; the 8-bit routine stores the bytes as computed */
+                R = R > mx ? mx : R; G = G > mx ? mx : G; B = B > mx ? mx : B;
+            }
+            put_rgb(rgb, w, x, y, bd, R, G, B);
+        }
+    }
+}
 
 /* ------------------------------------------------------------------ HQLI */
 /* process_hqli (hqli.cpp:7-44): 3x3 bilinear averages on [2,w-2)x[2,h-2)
@@ -667,10 +727,11 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, 
     free(L); free(M); free(Lm); free(Mm);
 }
 
-/* demosaic_cpu (dispatch.cpp:115-128) restricted to HQLI / MG / PRISM / RCD:
+/* demosaic_cpu (dispatch.cpp:115-128) restricted to SUPER_FAST / HQLI / MG /
+ * PRISM / RCD:
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
- * (hqli.cpp:11, mg.cpp:12, prism.cpp:29, rcd.cpp:11).  Other algorithms
+ * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, prism.cpp:29, rcd.cpp:11).  Other algorithms
  * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
@@ -679,6 +740,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (pattern < 0 || pattern > 3) return E_PATTERN;
     int min_dim;
     if (algo == A_HQLI || algo == A_MG) min_dim = 6;
+    else if (algo == A_SUPER_FAST) min_dim = 2;
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
@@ -686,7 +748,8 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     int *P = (int *)malloc((size_t)w * h * sizeof(int));
     if (!P) return E_INTERNAL;
     orc_decode_plane(in, w, h, bd, packed, P);
-    if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
+    if (algo == A_SUPER_FAST) super_fast(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
+    else if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);
         else mg_float(P, rgb, w, h, ph, bd);

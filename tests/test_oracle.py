@@ -206,6 +206,46 @@ def test_rcd_matches_compiled_reference(bd, packed, pattern):
 
 
 @pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (8, True), (10, True), (12, True), (16, False), (6, False)])
+def test_super_fast_matches_compiled_reference(bd, packed):
+    """SUPER_FAST (super_fast.cpp:9-43, avx2.cpp:99-253) against the compiled
+    reference: widths around the 8-bit routine's 32-pixel vectors, every
+    pattern, with and without black level; bytes above a shallow depth's
+    maximum."""
+    rng = np.random.default_rng(bd * 3 + packed)
+    for pattern in range(4):
+        for (h, w) in ((2, 2), (3, 5), (17, 31), (20, 32), (9, 33), (16, 64), (12, 70), (7, 95)):
+            vals = rng.integers(0, 1 << bd, size=(h, w))
+            raw = scene.raw_bytes(vals, bd, packed)
+            for bl in (None, [float(rng.integers(0, 40))] * 4):
+                e1, o1 = O.demosaic(raw, w, h, pattern, O.SUPER_FAST, bd, packed, bl)
+                e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.SUPER_FAST, bd, packed, bl)
+                assert e1 == e2 == 0
+                np.testing.assert_array_equal(o1, o2)
+    if bd == 6 and not packed:
+        raw = rng.integers(0, 256, size=(9, 64)).astype(np.uint8).ravel()
+        np.testing.assert_array_equal(O.demosaic(raw, 64, 9, 1, O.SUPER_FAST, 6)[1],
+                                      O.ref_demosaic(raw, 64, 9, 1, O.SUPER_FAST, 6)[1])
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+def test_planted_super_fast_scalar_core_is_detected():
+    """The scalar core (exact neighbours everywhere) differs from the reference
+    on an AVX2 host's 8-bit frames: the vector routine's edge operands are
+    pinned."""
+    lib = O.oracle()
+    lib.orc_set_super_fast_scalar.argtypes = [ctypes.c_int]
+    raw = np.random.default_rng(1).integers(0, 256, size=(8, 64)).astype(np.uint8).ravel()
+    want = O.ref_demosaic(raw, 64, 8, 0, O.SUPER_FAST, 8)[1]
+    np.testing.assert_array_equal(O.demosaic(raw, 64, 8, 0, O.SUPER_FAST, 8)[1], want)
+    lib.orc_set_super_fast_scalar(1)
+    try:
+        assert not np.array_equal(O.demosaic(raw, 64, 8, 0, O.SUPER_FAST, 8)[1], want)
+    finally:
+        lib.orc_set_super_fast_scalar(0)
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
 def test_planted_rcd_mistakes_are_detected():
     """RCD with PRISM's Laplacian term kept, and RCD without GCC's FMA
     contraction, both differ from the compiled reference: the oracle's
