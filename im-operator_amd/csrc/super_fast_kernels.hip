@@ -14,8 +14,10 @@
 // maximum (set_rgb_clamped).
 //
 // One workgroup: 4 rows x 256 pixels; the samples (decoded, black level
-// applied) of the block plus a 1-pixel ring are staged in LDS, then each
-// thread writes 4 consecutive pixels of one row.
+// applied) of the block plus a 1-pixel ring are staged in LDS; each thread
+// computes 4 consecutive pixels of one row into an LDS copy of the row's RGB
+// bytes, which its wave then writes as contiguous 16-byte non-temporal
+// stores (byte stores where the row segment does not start on 16 bytes).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -30,8 +32,10 @@ constexpr int LW = SW + 2, LH = SH + 2;      // staged samples: x0-1 .. x0+SW, y
 
 __global__ __launch_bounds__(256) void k_super_fast(FrameParams f) {
     __shared__ uint16_t s[LH][LW];
+    __shared__ __attribute__((aligned(16))) uint8_t o[SH][SW * 6];  // RGB bytes of each row segment
     const int w = f.w, h = f.h;
     const int x0 = blockIdx.x * SW, y0 = blockIdx.y * SH;
+    static_assert(SH * 64 == 256, "one wave per row");
     for (int k = threadIdx.x; k < LH * LW; k += blockDim.x) {
         const int j = k / LW, i = k - j * LW;
         const int x = min(max(x0 - 1 + i, 0), w - 1), y = min(max(y0 - 1 + j, 0), h - 1);
@@ -40,8 +44,7 @@ __global__ __launch_bounds__(256) void k_super_fast(FrameParams f) {
     __syncthreads();
 
     const int row = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int y = y0 + row;
-    if (y >= h) return;
+    const int y = min(y0 + row, h - 1);  // rows past the frame compute a copy of the last and store nothing
     const int full = (w / 32) * 32;  // pixels the 8-bit routine covers with 32-pixel vectors
     const bool gr_row = (y & 1) == r_row_of(f.pattern);
     const int ny = min(max((y & 1) ? y - 1 : y + 1, 0), h - 1);
@@ -77,8 +80,28 @@ __global__ __launch_bounds__(256) void k_super_fast(FrameParams f) {
             G = min(G, f.maxv);
             B = min(B, f.maxv);
         }
-        put_px(f, x, y, R, G, B);
+        if (f.out16) {
+            uint16_t* q = (uint16_t*)&o[row][(x - x0) * 6];
+            q[0] = (uint16_t)R; q[1] = (uint16_t)G; q[2] = (uint16_t)B;
+        } else {
+            uint8_t* q = &o[row][(x - x0) * 3];
+            q[0] = (uint8_t)R; q[1] = (uint8_t)G; q[2] = (uint8_t)B;
+        }
     }
+    __syncthreads();
+    if (y0 + row >= h) return;
+    const int es = f.out16 ? 6 : 3;
+    const int nbytes = min(SW, w - x0) * es;
+    uint8_t* dst = f.out + ((uint64_t)y * (uint64_t)w + (uint64_t)x0) * (uint64_t)es;
+    int done = 0;
+    if (((uintptr_t)dst & 15u) == 0) {
+        const int chunks = nbytes >> 4;
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        for (int c = lane; c < chunks; c += 64)
+            __builtin_nontemporal_store(*(const u32x4*)&o[row][16 * c], (u32x4*)(dst + 16 * c));
+        done = chunks << 4;
+    }
+    for (int b = done + lane; b < nbytes; b += 64) dst[b] = o[row][b];
 }
 
 }  // namespace
