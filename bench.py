@@ -37,6 +37,7 @@ WORKLOADS = {
     "prism10": (4024, 3032, 10, True, 8, 64.0, 3, 200),  # PRISM, 200 diagonal lines
     "rcd10": (4024, 3032, 10, True, 7, 64.0, 3, 200),    # RCD on the same frame
     "sf12": (4024, 3032, 12, True, 0, 64.0, 2, 0),       # SUPER_FAST on the MG frame
+    "l7_12": (4024, 3032, 12, True, 3, 64.0, 2, 0),      # L7 on the MG frame
     "hqli8": (1006, 758, 8, False, 1, None, 1, 0),
     "hqli12": (4024, 3032, 12, True, 1, 64.0, 2, 0),
 }
@@ -770,7 +771,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "rcd", "sf"), default=None,
+    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "rcd", "sf", "l7"), default=None,
                     help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
                          "and print its JSON object (for profiling)")
     ap.add_argument("--batch-shape", default=None,
@@ -812,8 +813,8 @@ def main():
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
-    if args.only in ("rcd", "sf"):
-        out = bench_single("rcd10" if args.only == "rcd" else "sf12", args.steps, args.warmup)
+    if args.only in ("rcd", "sf", "l7"):
+        out = bench_single({"rcd": "rcd10", "sf": "sf12", "l7": "l7_12"}[args.only], args.steps, args.warmup)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -953,6 +954,7 @@ def main():
         result["prism"] = bench_single("prism10", 20, 3)
         result["rcd"] = bench_single("rcd10", 20, 3)
         result["super_fast"] = bench_single("sf12", 50, 5)
+        result["l7"] = bench_single("l7_12", 50, 5)
         result["hqli_small"] = bench_single("hqli8", 200, 20)
     if not args.no_chain and args.workload == "mg12":
         torch.cuda.empty_cache()

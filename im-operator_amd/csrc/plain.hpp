@@ -23,6 +23,9 @@ hipError_t launch_black_level(uint8_t* data, int w, int h, int bd, const float q
 // SUPER_FAST for one frame (super_fast_kernels.hip).
 hipError_t launch_super_fast(const FrameParams& f, hipStream_t st);
 
+// L7 for one frame, interior then the width-3 border fill (l7_kernels.hip).
+hipError_t launch_l7(const FrameParams& f, hipStream_t st);
+
 // PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
 size_t prism_workspace_bytes(int w, int h);
 hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st, bool rcd = false);

@@ -20,6 +20,7 @@
  *   - PRISM (edge-directed)             src/algorithms/prism.cpp:25-316
  *   - SUPER_FAST (nearest / 2-4 average) src/algorithms/super_fast.cpp:9-43,
  *                                       avx2.cpp:99-253
+ *   - L7 (7x7 distance-weighted)        src/algorithms/l7.cpp:6-128, avx2.cpp:362-420
  *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
  *                                       stages S0-S6 without the Laplacian
  *                                       term, then borders and output)
@@ -308,6 +309,42 @@ static void super_fast(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int
             put_rgb(rgb, w, x, y, bd, R, G, B);
         }
     }
+}
+
+/* -------------------------------------------------------------------- L7 */
+/* process_l7 (l7.cpp:65-128): inside [3, w-3) x [3, h-3), each colour the
+ * pixel lacks is the rounded integer mean (sum + wsum / 2) / wsum of that
+ * colour's samples in the 7x7 window, weighted by kL7Weights7
+ * (l7.cpp:6-14: (4 - max(|dx|, |dy|))^2 ... the table itself; centre 0); the
+ * pixel's own colour is its sample.  8-bit unpacked frames (l7_avx2_8bit /
+ * l7_core_8bit, the same arithmetic) clamp to [0, 255]; the generic path to
+ * [0, max].  Then fill_rgb_borders with width 3. */
+static const int l7_weights[7][7] = {
+    {1, 4, 9, 16, 9, 4, 1},   {4, 9, 16, 25, 16, 9, 4},   {9, 16, 25, 36, 25, 16, 9}, {16, 25, 36, 0, 36, 25, 16},
+    {9, 16, 25, 36, 25, 16, 9}, {4, 9, 16, 25, 16, 9, 4}, {1, 4, 9, 16, 9, 4, 1}};
+
+static void l7(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, int byte_path) {
+    const int hi = byte_path ? 255 : orc_max_val(bd);
+    for (int y = 3; y < h - 3; ++y)
+        for (int x = 3; x < w - 3; ++x) {
+            int sum[3] = {0, 0, 0}, ws[3] = {0, 0, 0};
+            for (int dy = -3; dy <= 3; ++dy)
+                for (int dx = -3; dx <= 3; ++dx) {
+                    const int wv = l7_weights[dy + 3][dx + 3];
+                    if (wv == 0) continue;
+                    const int c = site(&ph, y + dy, x + dx);
+                    sum[c] += P[(size_t)(y + dy) * w + (x + dx)] * wv;
+                    ws[c] += wv;
+                }
+            const int own = site(&ph, y, x), v = P[(size_t)y * w + x];
+            int o[3];
+            for (int c = 0; c < 3; ++c) {
+                o[c] = c == own ? v : (ws[c] > 0 ? (sum[c] + ws[c] / 2) / ws[c] : 0);
+                o[c] = o[c] < 0 ? 0 : (o[c] > hi ? hi : o[c]);
+            }
+            put_rgb(rgb, w, x, y, bd, o[0], o[1], o[2]);
+        }
+    rgb_borders(rgb, w, h, bd, 3);
 }
 
 /* ------------------------------------------------------------------ HQLI */
@@ -728,10 +765,11 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, 
 }
 
 /* demosaic_cpu (dispatch.cpp:115-128) restricted to SUPER_FAST / HQLI / MG /
- * PRISM / RCD:
+ * L7 / PRISM / RCD:
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
- * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, prism.cpp:29, rcd.cpp:11).  Other algorithms
+ * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, prism.cpp:29,
+ * rcd.cpp:11).  Other algorithms
  * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
@@ -741,6 +779,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     int min_dim;
     if (algo == A_HQLI || algo == A_MG) min_dim = 6;
     else if (algo == A_SUPER_FAST) min_dim = 2;
+    else if (algo == A_L7) min_dim = 8;
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
@@ -749,6 +788,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (!P) return E_INTERNAL;
     orc_decode_plane(in, w, h, bd, packed, P);
     if (algo == A_SUPER_FAST) super_fast(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
+    else if (algo == A_L7) l7(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);

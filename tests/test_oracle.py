@@ -229,6 +229,37 @@ def test_super_fast_matches_compiled_reference(bd, packed):
 
 
 @pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (8, True), (10, True), (12, True), (16, False), (6, False)])
+def test_l7_matches_compiled_reference(bd, packed):
+    """L7 (l7.cpp:65-128, avx2.cpp:362-420) against the compiled reference:
+    every pattern, sizes from the 8 x 8 minimum, with and without black level."""
+    rng = np.random.default_rng(bd * 5 + packed)
+    for pattern in range(4):
+        for (h, w) in ((8, 8), (9, 13), (17, 31), (20, 40)):
+            vals = rng.integers(0, 1 << bd, size=(h, w))
+            raw = scene.raw_bytes(vals, bd, packed)
+            for bl in (None, [float(rng.integers(0, 40))] * 4):
+                e1, o1 = O.demosaic(raw, w, h, pattern, O.L7, bd, packed, bl)
+                e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.L7, bd, packed, bl)
+                assert e1 == e2 == 0
+                np.testing.assert_array_equal(o1, o2)
+    z = np.zeros(64, np.uint8)
+    assert O.demosaic(z, 7, 9, 0, O.L7, 8)[0] == O.ref_demosaic(z, 7, 9, 0, O.L7, 8)[0] == 5
+
+
+def test_l7_closed_forms():
+    """A constant frame gives the constant inside [3, w-3) x [3, h-3) and on the
+    copied borders, 0 at the four 3 x 3 corners; the weights' centre is 0, so
+    a single bright pixel does not reach its own colour's interpolation."""
+    w, h, v = 20, 14, 1234
+    raw = scene.raw_bytes(np.full((h, w), v, np.uint16), 12, False)
+    rgb = O.demosaic(raw, w, h, 0, O.L7, 12)[1]
+    corners = np.zeros((h, w), bool)
+    corners[:3, :3] = corners[:3, -3:] = corners[-3:, :3] = corners[-3:, -3:] = True
+    assert (rgb[~corners] == v).all() and (rgb[corners] == 0).all()
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
 def test_planted_super_fast_scalar_core_is_detected():
     """The scalar core (exact neighbours everywhere) differs from the reference
     on an AVX2 host's 8-bit frames: the vector routine's edge operands are
