@@ -45,7 +45,8 @@ int check_args(const void* in, const void* out, int w, int h, int pattern, int a
     if (bd < 1 || bd > 16) return kBitDepth;
     if (pattern < 0 || pattern > 3) return kPattern;
     if (w < min_dim_of(algo) || h < min_dim_of(algo)) return kSmall;
-    if (algo != kSuperFast && algo != kHqli && algo != kMg && algo != kL7 && algo != kPrism && algo != kRcd)
+    if (algo != kSuperFast && algo != kHqli && algo != kMg && algo != kL7 && algo != kAhd && algo != kPrism &&
+        algo != kRcd)
         return kInternal;
     // The reference reads packed streams below 8 bits as one byte per pixel
     // (pixel::get_raw), past the end of the stream; leave those to it.
@@ -156,6 +157,8 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
         he = launch_super_fast(f, st);
     } else if (algo == kL7) {
         he = launch_l7(f, st);
+    } else if (algo == kAhd) {
+        he = launch_ahd(f, st);
     } else if (algo == kPrism || algo == kRcd) {
         // the PRISM / RCD passes index their planes with 32-bit offsets
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;

@@ -26,6 +26,9 @@ hipError_t launch_super_fast(const FrameParams& f, hipStream_t st);
 // L7 for one frame, interior then the width-3 border fill (l7_kernels.hip).
 hipError_t launch_l7(const FrameParams& f, hipStream_t st);
 
+// AHD for one frame (ahd_kernels.hip).
+hipError_t launch_ahd(const FrameParams& f, hipStream_t st);
+
 // PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
 size_t prism_workspace_bytes(int w, int h);
 hipError_t launch_prism(const FrameParams& f, void* ws, hipStream_t st, bool rcd = false);

@@ -29,6 +29,7 @@
 // initial value (VH = PQ = 0.5, low-pass 0, chroma natives / 0 outside
 // [4, h-4) x [4, w-4)) keep those values.
 #include "device_common.hpp"
+#include "median9.hpp"
 #include "plain.hpp"
 
 #pragma clang fp contract(off)
@@ -201,25 +202,6 @@ __device__ __forceinline__ float stage_s6(Xf X, Gf G, float disc) {
     return clamp01(fmaf(1.0f - disc, vEst, fmaf(disc, hEst, g0)));
 }
 
-// prism_detail::median_3x3 (prism.cpp:9-21): the same exchange network.  (It
-// is not a true median network -- it fails the 0-1 principle -- so it is kept
-// exactly.)  Each exchange "if (a > b) swap" is min / max: with no NaNs the
-// two differ only in which sign a +-0 tie keeps, and a signed zero selected
-// here leaves g + median, the clamp and the rounding unchanged.
-__device__ __forceinline__ float median9(float v[9]) {
-#define IMOP_CX(a, b)                        \
-    {                                        \
-        const float ta = v[a], tb = v[b];    \
-        v[a] = fminf(ta, tb);                \
-        v[b] = fmaxf(ta, tb);                \
-    }
-    IMOP_CX(0, 1) IMOP_CX(3, 4) IMOP_CX(6, 7) IMOP_CX(1, 2) IMOP_CX(4, 5) IMOP_CX(7, 8)
-    IMOP_CX(0, 1) IMOP_CX(3, 4) IMOP_CX(6, 7) IMOP_CX(0, 3) IMOP_CX(4, 7) IMOP_CX(1, 4)
-    IMOP_CX(3, 6) IMOP_CX(2, 5) IMOP_CX(5, 8) IMOP_CX(1, 3) IMOP_CX(4, 6) IMOP_CX(2, 5)
-    IMOP_CX(2, 3) IMOP_CX(5, 6) IMOP_CX(2, 4) IMOP_CX(3, 5) IMOP_CX(3, 4)
-#undef IMOP_CX
-    return v[4];
-}
 
 // LDS plane of a tile with apron A: element (x, y) of the frame.
 template <int A>

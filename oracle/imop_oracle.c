@@ -21,6 +21,7 @@
  *   - SUPER_FAST (nearest / 2-4 average) src/algorithms/super_fast.cpp:9-43,
  *                                       avx2.cpp:99-253
  *   - L7 (7x7 distance-weighted)        src/algorithms/l7.cpp:6-128, avx2.cpp:362-420
+ *   - AHD (adaptive homogeneity)        src/algorithms/ahd.cpp:7-187
  *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
  *                                       stages S0-S6 without the Laplacian
  *                                       term, then borders and output)
@@ -764,12 +765,118 @@ static void prism(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, 
     free(L); free(M); free(Lm); free(Mm);
 }
 
+/* ------------------------------------------------------------------- AHD */
+/* process_ahd (ahd.cpp:43-187): horizontal and vertical bilinear estimates
+ * (samples clamped into the frame; halves and quarters of integer sums,
+ * exact in float), their colour differences R - G and B - G, each filtered
+ * by the 3x3 exchange network of apply_median_filter_3x3 (ahd.cpp:7-40, the
+ * same network as prism_detail::median_3x3) on [1, w-1) x [1, h-1), then per
+ * pixel the direction whose filtered differences have the smaller sum of 3x3
+ * variances (coordinates clamped; mean = sum / 9, var = sum of diff^2 with
+ * GCC's contraction fmaf(diff, diff, var), / 9); strict < picks horizontal.
+ * Output (int)(v + 0.5f) clamped to [0, max]. */
+/* Planted mistakes for the tests: 1 = ties go horizontal (<=), 2 = no median. */
+static int g_ahd_mistake = 0;
+void orc_set_ahd_mistake(int k) { g_ahd_mistake = k; }
+
+static void ahd(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) {
+    const size_t n = (size_t)w * h;
+    const int mx = orc_max_val(bd);
+    float *pl[10];
+    for (int k = 0; k < 10; ++k) pl[k] = (float *)calloc(n, sizeof(float));
+    float *Rh = pl[0], *Gh = pl[1], *Bh = pl[2], *Rv = pl[3], *Gv = pl[4], *Bv = pl[5];
+    float *Lm[4] = {pl[6], pl[7], pl[8], pl[9]};  /* Lh, Mh, Lv, Mv */
+#define RAW(X, Y) ((float)P[(size_t)((Y) < 0 ? 0 : (Y) >= h ? h - 1 : (Y)) * w + (size_t)((X) < 0 ? 0 : (X) >= w ? w - 1 : (X))])
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const int s = site(&ph, y, x);
+            for (int dir = 0; dir < 2; ++dir) {
+                float r, g, b;
+                if (s == 1) {
+                    g = RAW(x, y);
+                    const float hz = (RAW(x - 1, y) + RAW(x + 1, y)) * 0.5f, vt = (RAW(x, y - 1) + RAW(x, y + 1)) * 0.5f;
+                    /* the chroma whose samples sit beside the pixel along dir */
+                    const int r_beside = dir == 0 ? (site(&ph, y, x - 1) == 0 || site(&ph, y, x + 1) == 0)
+                                                  : (site(&ph, y - 1, x) == 0 || site(&ph, y + 1, x) == 0);
+                    const float along = dir == 0 ? hz : vt, across = dir == 0 ? vt : hz;
+                    r = r_beside ? along : across;
+                    b = r_beside ? across : along;
+                } else {
+                    const float own = RAW(x, y);
+                    g = dir == 0 ? (RAW(x - 1, y) + RAW(x + 1, y)) * 0.5f : (RAW(x, y - 1) + RAW(x, y + 1)) * 0.5f;
+                    const float diag = dir == 0
+                        ? (RAW(x - 1, y - 1) + RAW(x + 1, y - 1) + RAW(x - 1, y + 1) + RAW(x + 1, y + 1)) * 0.25f
+                        : (RAW(x - 1, y - 1) + RAW(x - 1, y + 1) + RAW(x + 1, y - 1) + RAW(x + 1, y + 1)) * 0.25f;
+                    r = s == 0 ? own : diag;
+                    b = s == 0 ? diag : own;
+                }
+                if (dir == 0) { Rh[i] = r; Gh[i] = g; Bh[i] = b; }
+                else { Rv[i] = r; Gv[i] = g; Bv[i] = b; }
+            }
+        }
+#undef RAW
+    for (size_t i = 0; i < n; ++i) {
+        Lm[0][i] = Rh[i] - Gh[i]; Lm[1][i] = Bh[i] - Gh[i];
+        Lm[2][i] = Rv[i] - Gv[i]; Lm[3][i] = Bv[i] - Gv[i];
+    }
+    float *filt = (float *)malloc(n * sizeof(float));
+    for (int k = 0; k < 4; ++k) {
+        float *p = Lm[k];
+        if (w >= 3 && h >= 3 && g_ahd_mistake != 2) {
+            for (int y = 1; y < h - 1; ++y)
+                for (int x = 1; x < w - 1; ++x) {
+                    float v[9];
+                    int q = 0;
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) v[q++] = p[(size_t)(y + dy) * w + (x + dx)];
+                    filt[(size_t)y * w + x] = median9(v);
+                }
+            for (int y = 1; y < h - 1; ++y)
+                for (int x = 1; x < w - 1; ++x) p[(size_t)y * w + x] = filt[(size_t)y * w + x];
+        }
+    }
+    free(filt);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float var[4];
+            for (int k = 0; k < 4; ++k) {
+                float mean = 0.0f, v = 0.0f;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int nx = x + dx, ny = y + dy;
+                        nx = nx < 0 ? 0 : (nx > w - 1 ? w - 1 : nx);
+                        ny = ny < 0 ? 0 : (ny > h - 1 ? h - 1 : ny);
+                        mean += Lm[k][(size_t)ny * w + nx];
+                    }
+                mean /= 9.0f;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int nx = x + dx, ny = y + dy;
+                        nx = nx < 0 ? 0 : (nx > w - 1 ? w - 1 : nx);
+                        ny = ny < 0 ? 0 : (ny > h - 1 ? h - 1 : ny);
+                        const float d = Lm[k][(size_t)ny * w + nx] - mean;
+                        v = mad(d, d, v);
+                    }
+                var[k] = v / 9.0f;
+            }
+            const size_t i = (size_t)y * w + x;
+            const int horiz = g_ahd_mistake == 1 ? (var[0] + var[1]) <= (var[2] + var[3])
+                                                 : (var[0] + var[1]) < (var[2] + var[3]);
+            const float fr = horiz ? Rh[i] : Rv[i], fg = horiz ? Gh[i] : Gv[i], fb = horiz ? Bh[i] : Bv[i];
+            int o[3] = {(int)(fr + 0.5f), (int)(fg + 0.5f), (int)(fb + 0.5f)};
+            for (int c = 0; c < 3; ++c) o[c] = o[c] < 0 ? 0 : (o[c] > mx ? mx : o[c]);
+            put_rgb(rgb, w, x, y, bd, o[0], o[1], o[2]);
+        }
+    for (int k = 0; k < 10; ++k) free(pl[k]);
+}
+
 /* demosaic_cpu (dispatch.cpp:115-128) restricted to SUPER_FAST / HQLI / MG /
- * L7 / PRISM / RCD:
+ * L7 / AHD / PRISM / RCD:
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
- * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, prism.cpp:29,
- * rcd.cpp:11).  Other algorithms
+ * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, ahd.cpp:46,
+ * prism.cpp:29, rcd.cpp:11).  Other algorithms
  * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
@@ -780,6 +887,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (algo == A_HQLI || algo == A_MG) min_dim = 6;
     else if (algo == A_SUPER_FAST) min_dim = 2;
     else if (algo == A_L7) min_dim = 8;
+    else if (algo == A_AHD) min_dim = 6;
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
@@ -789,6 +897,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     orc_decode_plane(in, w, h, bd, packed, P);
     if (algo == A_SUPER_FAST) super_fast(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_L7) l7(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
+    else if (algo == A_AHD) ahd(P, rgb, w, h, ph, bd);
     else if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);

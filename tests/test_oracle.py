@@ -247,6 +247,47 @@ def test_l7_matches_compiled_reference(bd, packed):
     assert O.demosaic(z, 7, 9, 0, O.L7, 8)[0] == O.ref_demosaic(z, 7, 9, 0, O.L7, 8)[0] == 5
 
 
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (8, True), (10, True), (12, True), (16, False), (6, False)])
+def test_ahd_matches_compiled_reference(bd, packed):
+    """AHD (ahd.cpp:43-187) against the compiled reference: scenes and noise,
+    every pattern, sizes from the 6 x 6 minimum, with and without black level."""
+    rng = np.random.default_rng(bd * 7 + packed)
+    for pattern in range(4):
+        for k, (h, w) in enumerate(((6, 6), (7, 9), (17, 31), (20, 40))):
+            vals = rng.integers(0, 1 << bd, size=(h, w)) if k % 2 else scene.make_scene(h, w, bd, pattern, seed=k)
+            raw = scene.raw_bytes(vals, bd, packed)
+            for bl in (None, [float(rng.integers(0, 40))] * 4):
+                e1, o1 = O.demosaic(raw, w, h, pattern, O.AHD, bd, packed, bl)
+                e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.AHD, bd, packed, bl)
+                assert e1 == e2 == 0
+                np.testing.assert_array_equal(o1, o2)
+    z = np.zeros(64, np.uint8)
+    assert O.demosaic(z, 5, 8, 0, O.AHD, 8)[0] == O.ref_demosaic(z, 5, 8, 0, O.AHD, 8)[0] == 5
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+def test_planted_ahd_mistakes_are_detected():
+    """Skipping the difference medians, or sending variance ties to the
+    horizontal estimate (<= for the reference's <), differs from the compiled
+    reference: the direction choice is pinned, not assumed."""
+    lib = O.oracle()
+    lib.orc_set_ahd_mistake.argtypes = [ctypes.c_int]
+    vals = scene.make_scene(48, 64, 12, 0, seed=0)
+    raw = scene.raw_bytes(vals, 12, True)
+    rng = np.random.default_rng(0)
+    rng.integers(0, 3, size=(8, 10))
+    stripes = np.repeat(rng.integers(0, 4, size=(8, 1)) * 60, 10, axis=1).astype(np.uint8).ravel()
+    for mistake, (r, w, h, pat, bd, packed) in ((2, (raw, 64, 48, 0, 12, True)), (1, (stripes, 10, 8, 1, 8, False))):
+        want = O.ref_demosaic(r, w, h, pat, O.AHD, bd, packed)[1]
+        np.testing.assert_array_equal(O.demosaic(r, w, h, pat, O.AHD, bd, packed)[1], want)
+        lib.orc_set_ahd_mistake(mistake)
+        try:
+            assert not np.array_equal(O.demosaic(r, w, h, pat, O.AHD, bd, packed)[1], want)
+        finally:
+            lib.orc_set_ahd_mistake(0)
+
+
 def test_l7_closed_forms():
     """A constant frame gives the constant inside [3, w-3) x [3, h-3) and on the
     copied borders, 0 at the four 3 x 3 corners; the weights' centre is 0, so
