@@ -45,9 +45,7 @@ int check_args(const void* in, const void* out, int w, int h, int pattern, int a
     if (bd < 1 || bd > 16) return kBitDepth;
     if (pattern < 0 || pattern > 3) return kPattern;
     if (w < min_dim_of(algo) || h < min_dim_of(algo)) return kSmall;
-    if (algo != kSuperFast && algo != kHqli && algo != kMg && algo != kL7 && algo != kAhd && algo != kPrism &&
-        algo != kRcd)
-        return kInternal;
+    if (algo == kDfpd || algo < kSuperFast || algo > kPrism) return kInternal;
     // The reference reads packed streams below 8 bits as one byte per pixel
     // (pixel::get_raw), past the end of the stream; leave those to it.
     if (packed && bd < 8) return kInternal;
@@ -159,6 +157,13 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
         he = launch_l7(f, st);
     } else if (algo == kAhd) {
         he = launch_ahd(f, st);
+    } else if (algo == kAmaze) {
+        void* ws = prism_ws;
+        if (!ws) {
+            he = get_scratch(amaze_workspace_bytes(f.w, f.h), &ws);
+            if (he != hipSuccess) return kInternal;
+        }
+        he = launch_amaze(f, ws, st);
     } else if (algo == kPrism || algo == kRcd) {
         // the PRISM / RCD passes index their planes with 32-bit offsets
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;

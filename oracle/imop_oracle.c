@@ -22,6 +22,8 @@
  *                                       avx2.cpp:99-253
  *   - L7 (7x7 distance-weighted)        src/algorithms/l7.cpp:6-128, avx2.cpp:362-420
  *   - AHD (adaptive homogeneity)        src/algorithms/ahd.cpp:7-187
+ *   - AMAZE (gradient-weighted green)   src/algorithms/amaze.cpp:7-203,
+ *                                       common.hpp:143-214
  *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
  *                                       stages S0-S6 without the Laplacian
  *                                       term, then borders and output)
@@ -871,12 +873,96 @@ static void ahd(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) {
     for (int k = 0; k < 10; ++k) free(pl[k]);
 }
 
+/* ----------------------------------------------------------------- AMAZE */
+/* process_amaze (amaze.cpp:158-200) and its 8-bit core (amaze.cpp:7-156):
+ *   G plane starts at -1 everywhere; inside [2, w-2) x [2, h-2) it is the
+ *   sample at G sites, else the inverse-gradient blend of the horizontal and
+ *   vertical estimates (each with a quarter Laplacian), fmaf(wh, gh, wv * gv)
+ *   as GCC contracts `wh * gh + wv * gv`;
+ *   inside [1, w-1) x [1, h-1): the chroma at R/B sites from the diagonal
+ *   opposite-colour differences, at G sites from the cross neighbours
+ *   (interpolate_rb_at_*_positions, common.hpp:143-199), every difference
+ *   against the G plane as it stands, sentinel included; R = B = 0 elsewhere;
+ *   fill_intermediate_borders(2), then (int)(v + 0.5f) clamped to [0, max]
+ *   (the 8-bit core: [0, 255], then fill_rgb_borders(2) on the output). */
+static void amaze(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, int byte_path) {
+    const size_t n = (size_t)w * h;
+    float *G = (float *)malloc(n * sizeof(float)), *R = (float *)calloc(n, sizeof(float)),
+          *B = (float *)calloc(n, sizeof(float));
+    for (size_t i = 0; i < n; ++i) G[i] = -1.0f;
+#define RW(X, Y) ((float)P[(size_t)(Y) * w + (size_t)(X)])
+    for (int y = 2; y < h - 2; ++y)
+        for (int x = 2; x < w - 2; ++x) {
+            const size_t i = (size_t)y * w + x;
+            if (site(&ph, y, x) == 1) { G[i] = RW(x, y); continue; }
+            const float c = RW(x, y);
+            const float gH = fabsf(RW(x - 1, y) - RW(x + 1, y)) + fabsf(2.0f * c - RW(x - 2, y) - RW(x + 2, y));
+            const float gV = fabsf(RW(x, y - 1) - RW(x, y + 1)) + fabsf(2.0f * c - RW(x, y - 2) - RW(x, y + 2));
+            float wh = 1.0f / (1.0f + gH), wv = 1.0f / (1.0f + gV);
+            const float sw = wh + wv;
+            wh /= sw;
+            wv /= sw;
+            float gh = (RW(x - 1, y) + RW(x + 1, y)) * 0.5f, gv = (RW(x, y - 1) + RW(x, y + 1)) * 0.5f;
+            const float lH = 2.0f * c - RW(x - 2, y) - RW(x + 2, y), lV = 2.0f * c - RW(x, y - 2) - RW(x, y + 2);
+            gh += lH * 0.25f;
+            gv += lV * 0.25f;
+            G[i] = mad(wh, gh, wv * gv);
+        }
+    for (int y = 1; y < h - 1; ++y)
+        for (int x = 1; x < w - 1; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const float g = G[i];
+            const int s = site(&ph, y, x);
+            if (s != 1) {
+                float sum = 0.0f;
+                int cnt = 0;
+                static const int dg[4][2] = {{-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+                for (int k = 0; k < 4; ++k) {
+                    const int nx = x + dg[k][0], ny = y + dg[k][1];
+                    if (nx >= 0 && nx < w && ny >= 0 && ny < h && site(&ph, ny, nx) == 2 - s) {
+                        sum += RW(nx, ny) - G[(size_t)ny * w + nx];
+                        cnt++;
+                    }
+                }
+                const float other = g + (cnt > 0 ? sum / (float)cnt : 0.0f);
+                if (s == 0) { R[i] = RW(x, y); B[i] = other; }
+                else { B[i] = RW(x, y); R[i] = other; }
+            } else {
+                float rs = 0.0f, bs = 0.0f;
+                int rc = 0, bc = 0;
+                static const int cr[4][2] = {{0, -1}, {0, 1}, {-1, 0}, {1, 0}};
+                for (int k = 0; k < 4; ++k) {
+                    const int nx = x + cr[k][0], ny = y + cr[k][1];
+                    if (site(&ph, ny, nx) == 0) { rs += RW(nx, ny) - G[(size_t)ny * w + nx]; rc++; }
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const int nx = x + cr[k][0], ny = y + cr[k][1];
+                    if (site(&ph, ny, nx) == 2) { bs += RW(nx, ny) - G[(size_t)ny * w + nx]; bc++; }
+                }
+                R[i] = g + (rc > 0 ? rs / (float)rc : 0.0f);
+                B[i] = g + (bc > 0 ? bs / (float)bc : 0.0f);
+            }
+        }
+#undef RW
+    plane_borders_f(R, G, B, w, h, 2);
+    const int mx = byte_path ? 255 : orc_max_val(bd);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            int o[3] = {(int)(R[i] + 0.5f), (int)(G[i] + 0.5f), (int)(B[i] + 0.5f)};
+            for (int c = 0; c < 3; ++c) o[c] = o[c] < 0 ? 0 : (o[c] > mx ? mx : o[c]);
+            put_rgb(rgb, w, x, y, bd, o[0], o[1], o[2]);
+        }
+    if (byte_path) rgb_borders(rgb, w, h, bd, 2);
+    free(G); free(R); free(B);
+}
+
 /* demosaic_cpu (dispatch.cpp:115-128) restricted to SUPER_FAST / HQLI / MG /
- * L7 / AHD / PRISM / RCD:
+ * L7 / AHD / AMAZE / PRISM / RCD:
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
  * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, ahd.cpp:46,
- * prism.cpp:29, rcd.cpp:11).  Other algorithms
+ * amaze.cpp:162, prism.cpp:29, rcd.cpp:11).  Other algorithms
  * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
@@ -887,7 +973,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (algo == A_HQLI || algo == A_MG) min_dim = 6;
     else if (algo == A_SUPER_FAST) min_dim = 2;
     else if (algo == A_L7) min_dim = 8;
-    else if (algo == A_AHD) min_dim = 6;
+    else if (algo == A_AHD || algo == A_AMAZE) min_dim = 6;
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
@@ -898,6 +984,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (algo == A_SUPER_FAST) super_fast(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_L7) l7(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_AHD) ahd(P, rgb, w, h, ph, bd);
+    else if (algo == A_AMAZE) amaze(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);

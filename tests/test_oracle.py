@@ -288,6 +288,47 @@ def test_planted_ahd_mistakes_are_detected():
             lib.orc_set_ahd_mistake(0)
 
 
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (8, True), (10, True), (12, True), (16, False), (6, False)])
+def test_amaze_matches_compiled_reference(bd, packed):
+    """AMAZE (amaze.cpp:7-200, common.hpp:143-214) against the compiled
+    reference: the 8-bit core and the generic path, every pattern, sizes from
+    the 6 x 6 minimum, scenes and noise, with and without black level."""
+    rng = np.random.default_rng(bd * 11 + packed)
+    for pattern in range(4):
+        for k, (h, w) in enumerate(((6, 6), (7, 9), (17, 31), (20, 40))):
+            vals = rng.integers(0, 1 << bd, size=(h, w)) if k % 2 else scene.make_scene(h, w, bd, pattern, seed=k)
+            raw = scene.raw_bytes(vals, bd, packed)
+            for bl in (None, [float(rng.integers(0, 40))] * 4):
+                e1, o1 = O.demosaic(raw, w, h, pattern, O.AMAZE, bd, packed, bl)
+                e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.AMAZE, bd, packed, bl)
+                assert e1 == e2 == 0
+                np.testing.assert_array_equal(o1, o2)
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+def test_planted_amaze_fma_is_detected():
+    """AMAZE without GCC's contraction of the green blend differs from the
+    compiled reference on 16-bit noise: the fused site is pinned."""
+    lib = O.oracle()
+    lib.orc_set_prism_fma.argtypes = [ctypes.c_int]
+    vals = np.random.default_rng(0).integers(0, 65536, size=(48, 64))
+    raw = scene.raw_bytes(vals, 16, False)
+    want = O.ref_demosaic(raw, 64, 48, 0, O.AMAZE, 16, False)[1]
+    np.testing.assert_array_equal(O.demosaic(raw, 64, 48, 0, O.AMAZE, 16, False)[1], want)
+    diffs = 0
+    lib.orc_set_prism_fma(0)
+    try:
+        for seed in range(4):
+            v = np.random.default_rng(seed).integers(0, 65536, size=(48, 64))
+            r = scene.raw_bytes(v, 16, False)
+            diffs += int(np.count_nonzero(O.demosaic(r, 64, 48, 0, O.AMAZE, 16, False)[1]
+                                          != O.ref_demosaic(r, 64, 48, 0, O.AMAZE, 16, False)[1]))
+    finally:
+        lib.orc_set_prism_fma(1)
+    assert diffs > 0
+
+
 def test_l7_closed_forms():
     """A constant frame gives the constant inside [3, w-3) x [3, h-3) and on the
     copied borders, 0 at the four 3 x 3 corners; the weights' centre is 0, so
