@@ -40,6 +40,7 @@ WORKLOADS = {
     "l7_12": (4024, 3032, 12, True, 3, 64.0, 2, 0),      # L7 on the MG frame
     "ahd12": (4024, 3032, 12, True, 5, 64.0, 2, 0),      # AHD on the MG frame
     "amaze12": (4024, 3032, 12, True, 6, 64.0, 2, 0),    # AMAZE on the MG frame
+    "dfpd12": (4024, 3032, 12, True, 4, 64.0, 2, 0),     # DFPD on the MG frame
     "hqli8": (1006, 758, 8, False, 1, None, 1, 0),
     "hqli12": (4024, 3032, 12, True, 1, 64.0, 2, 0),
 }
@@ -773,7 +774,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "rcd", "sf", "l7", "ahd", "amaze"),
+    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "rcd", "sf", "l7", "ahd", "amaze", "dfpd"),
                     default=None,
                     help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
                          "and print its JSON object (for profiling)")
@@ -816,9 +817,9 @@ def main():
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
-    if args.only in ("rcd", "sf", "l7", "ahd", "amaze"):
-        out = bench_single({"rcd": "rcd10", "sf": "sf12", "l7": "l7_12", "ahd": "ahd12", "amaze": "amaze12"}[args.only],
-                           args.steps, args.warmup)
+    if args.only in ("rcd", "sf", "l7", "ahd", "amaze", "dfpd"):
+        names = {"rcd": "rcd10", "sf": "sf12", "l7": "l7_12", "ahd": "ahd12", "amaze": "amaze12", "dfpd": "dfpd12"}
+        out = bench_single(names[args.only], args.steps, args.warmup)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -961,6 +962,7 @@ def main():
         result["l7"] = bench_single("l7_12", 50, 5)
         result["ahd"] = bench_single("ahd12", 20, 3)
         result["amaze"] = bench_single("amaze12", 50, 5)
+        result["dfpd"] = bench_single("dfpd12", 50, 5)
         result["hqli_small"] = bench_single("hqli8", 200, 20)
     if not args.no_chain and args.workload == "mg12":
         torch.cuda.empty_cache()

@@ -45,7 +45,7 @@ int check_args(const void* in, const void* out, int w, int h, int pattern, int a
     if (bd < 1 || bd > 16) return kBitDepth;
     if (pattern < 0 || pattern > 3) return kPattern;
     if (w < min_dim_of(algo) || h < min_dim_of(algo)) return kSmall;
-    if (algo == kDfpd || algo < kSuperFast || algo > kPrism) return kInternal;
+    if (algo < kSuperFast || algo > kPrism) return kInternal;
     // The reference reads packed streams below 8 bits as one byte per pixel
     // (pixel::get_raw), past the end of the stream; leave those to it.
     if (packed && bd < 8) return kInternal;
@@ -155,6 +155,8 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
         he = launch_super_fast(f, st);
     } else if (algo == kL7) {
         he = launch_l7(f, st);
+    } else if (algo == kDfpd) {
+        he = launch_dfpd(f, st);
     } else if (algo == kAhd) {
         he = launch_ahd(f, st);
     } else if (algo == kAmaze) {

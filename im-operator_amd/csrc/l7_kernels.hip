@@ -17,7 +17,7 @@
 // with the row's parity and the CFA pattern fixed, the colour of every tap
 // and the weight sums are compile-time, so the window reduces to
 // multiply-adds and divisions by constants.  The RGB bytes go through LDS
-// to 16-byte non-temporal stores.  k_l7_borders then fills the border.
+// to 16-byte non-temporal stores.  k_rgb_borders then fills the border.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -174,25 +174,31 @@ __global__ __launch_bounds__(256) void k_l7(FrameParams f) {
     }
 }
 
-// fill_rgb_borders(rgb, w, h, bd, 3) on the output k_l7 wrote.
-__global__ __launch_bounds__(256) void k_l7_borders(FrameParams f) {
-    const int w = f.w, h = f.h, b = 3;
+// fill_rgb_borders(rgb, w, h, bd, b) (common.hpp:47-96) on an output whose
+// interior [m, w-m) x [m, h-m) (m <= b) is written and the rest never was
+// (0 here).  The reference copies whole rows first -- rows above b and below
+// h-1-b take row b / h-1-b as it stands, its never-written ends giving 0 --
+// and then the side columns of rows [b, h-b); ROWS selects that step, and
+// the two steps run as two launches in that order.
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_rgb_borders(FrameParams f, int b, int m) {
+    const int w = f.w, h = f.h;
     const uint64_t top = (uint64_t)b * (uint64_t)w;     // per band of rows (top, bottom)
     const uint64_t side = (uint64_t)(h - 2 * b) * (uint64_t)b;  // per side
-    const uint64_t n = 2u * top + 2u * side;
+    const uint64_t n = ROWS ? 2u * top : 2u * side;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         int x, y, sx, sy;
         bool zero = false;
-        if (i < 2u * top) {
+        if (ROWS) {
             const uint64_t k = i < top ? i : i - top;
             y = (int)(k / (uint64_t)w);
             x = (int)(k - (uint64_t)y * (uint64_t)w);
             if (i >= top) y += h - b;
             sy = i < top ? b : h - 1 - b;
             sx = x;
-            zero = x < b || x >= w - b;
+            zero = x < m || x >= w - m;
         } else {
-            const uint64_t k = i - 2u * top;
+            const uint64_t k = i;
             const uint64_t kk = k < side ? k : k - side;
             y = b + (int)(kk / (uint64_t)b);
             const int c = (int)(kk - (uint64_t)(y - b) * (uint64_t)b);
@@ -217,9 +223,17 @@ hipError_t launch_l7(const FrameParams& f, hipStream_t st) {
     const dim3 grid((unsigned)((f.w + SW - 1) / SW), (unsigned)((f.h + SH - 1) / SH)), block(256);
     if (grid.y > 65535u) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_l7, grid, block, 0, st, f);
-    const uint64_t n = 6u * (uint64_t)f.w + 6u * (uint64_t)(f.h - 6);
-    const uint64_t blocks = (n + 255u) / 256u;
-    hipLaunchKernelGGL(k_l7_borders, dim3((unsigned)(blocks < 4096u ? blocks : 4096u)), block, 0, st, f);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_rgb_borders(f, 3, 3, st);
+}
+
+hipError_t launch_rgb_borders(const FrameParams& f, int b, int m, hipStream_t st) {
+    if (b <= 0 || f.w <= 2 * b || f.h <= 2 * b) return hipSuccess;
+    const uint64_t rows = 2u * (uint64_t)b * (uint64_t)f.w, sides = 2u * (uint64_t)b * (uint64_t)(f.h - 2 * b);
+    const uint64_t br = (rows + 255u) / 256u, bs = (sides + 255u) / 256u;
+    hipLaunchKernelGGL(k_rgb_borders<true>, dim3((unsigned)(br < 4096u ? br : 4096u)), dim3(256), 0, st, f, b, m);
+    hipLaunchKernelGGL(k_rgb_borders<false>, dim3((unsigned)(bs < 4096u ? (bs ? bs : 1u) : 4096u)), dim3(256), 0, st,
+                       f, b, m);
     return hipGetLastError();
 }
 

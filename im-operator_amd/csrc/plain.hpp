@@ -25,6 +25,11 @@ hipError_t launch_super_fast(const FrameParams& f, hipStream_t st);
 
 // L7 for one frame, interior then the width-3 border fill (l7_kernels.hip).
 hipError_t launch_l7(const FrameParams& f, hipStream_t st);
+// fill_rgb_borders of width b on an output whose interior [m, w-m) x [m, h-m)
+// is written, m <= b (l7_kernels.hip).
+hipError_t launch_rgb_borders(const FrameParams& f, int b, int m, hipStream_t st);
+// DFPD for one frame, interior then the width-5 border fill (dfpd_kernels.hip).
+hipError_t launch_dfpd(const FrameParams& f, hipStream_t st);
 
 // AHD for one frame (ahd_kernels.hip).
 hipError_t launch_ahd(const FrameParams& f, hipStream_t st);

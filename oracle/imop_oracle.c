@@ -22,6 +22,7 @@
  *                                       avx2.cpp:99-253
  *   - L7 (7x7 distance-weighted)        src/algorithms/l7.cpp:6-128, avx2.cpp:362-420
  *   - AHD (adaptive homogeneity)        src/algorithms/ahd.cpp:7-187
+ *   - DFPD (directional green, chroma)  src/algorithms/dfpd.cpp:7-263
  *   - AMAZE (gradient-weighted green)   src/algorithms/amaze.cpp:7-203,
  *                                       common.hpp:143-214
  *   - RCD (ratio-corrected directional) src/algorithms/rcd.cpp:7-248 (PRISM's
@@ -873,6 +874,116 @@ static void ahd(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd) {
     for (int k = 0; k < 10; ++k) free(pl[k]);
 }
 
+/* ------------------------------------------------------------------ DFPD */
+/* process_dfpd (dfpd.cpp:146-263) and its 8-bit core (dfpd.cpp:7-144): green
+ * on [2, w-2) x [2, h-2) along the direction of smaller gradient (the
+ * horizontal / vertical average plus half the Laplacian correction; the mean
+ * of both on a tie), then on [3, w-3) x [3, h-3) the chroma from the mean
+ * diagonal (R/B sites, 4 terms) or cross (G sites, 2 terms) colour
+ * differences.  The 8-bit core works in integers: (a + b + 1) >> 1 halves,
+ * (sum + cnt / 2) / cnt means (C division), clamp to [0, 255].  The generic
+ * path works in float, where every value is a multiple of 1/16 and exact:
+ * (int)(v + 0.5f) and set_rgb_clamped.  Green is only ever needed where it
+ * was computed, so interp_g's recomputation of a sentinel never applies.
+ * Then fill_rgb_borders with width 5. */
+static void dfpd(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, int byte_path) {
+    const size_t n = (size_t)w * h;
+    const int mx = byte_path ? 255 : orc_max_val(bd);
+    float *G = (float *)calloc(n, sizeof(float)); /* 8-bit: integer values */
+#define RV(X, Y) (P[(size_t)(Y) * w + (size_t)(X)])
+    for (int y = 2; y < h - 2; ++y)
+        for (int x = 2; x < w - 2; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const int c = RV(x, y);
+            if (site(&ph, y, x) == 1) { G[i] = (float)c; continue; }
+            const int xm1 = RV(x - 1, y), xp1 = RV(x + 1, y), xm2 = RV(x - 2, y), xp2 = RV(x + 2, y);
+            const int ym1 = RV(x, y - 1), yp1 = RV(x, y + 1), ym2 = RV(x, y - 2), yp2 = RV(x, y + 2);
+            const int gh = abs(xm1 - xp1) + abs(2 * c - xm2 - xp2), gv = abs(ym1 - yp1) + abs(2 * c - ym2 - yp2);
+            if (byte_path) {
+                const int g_h = (xm1 + xp1 + 1) >> 1, g_v = (ym1 + yp1 + 1) >> 1;
+                int g;
+                if (gh < gv) g = g_h + ((c - ((xm2 + xp2 + 1) >> 1) + 1) >> 1);
+                else if (gv < gh) g = g_v + ((c - ((ym2 + yp2 + 1) >> 1) + 1) >> 1);
+                else g = (g_h + g_v + 1) >> 1;
+                G[i] = (float)g;
+            } else {
+                const float g_h = (float)(xm1 + xp1) * 0.5f, g_v = (float)(ym1 + yp1) * 0.5f;
+                if (gh < gv) G[i] = g_h + ((float)c - (float)(xm2 + xp2) * 0.5f) * 0.5f;
+                else if (gv < gh) G[i] = g_v + ((float)c - (float)(ym2 + yp2) * 0.5f) * 0.5f;
+                else G[i] = (g_h + g_v) * 0.5f;
+            }
+        }
+    for (int y = 3; y < h - 3; ++y)
+        for (int x = 3; x < w - 3; ++x) {
+            const size_t i = (size_t)y * w + x;
+            const int s = site(&ph, y, x);
+            const float g = G[i];
+            int R, Gc, B;
+            static const int dg[4][2] = {{-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+            static const int cr[4][2] = {{0, -1}, {0, 1}, {-1, 0}, {1, 0}};
+            if (byte_path) {
+                const int gi = (int)g;
+                if (s != 1) {
+                    int sum = 0, cnt = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + dg[k][0], ny = y + dg[k][1];
+                        if (site(&ph, ny, nx) == 2 - s) { sum += RV(nx, ny) - (int)G[(size_t)ny * w + nx]; cnt++; }
+                    }
+                    const int other = cnt > 0 ? gi + (sum + cnt / 2) / cnt : gi;
+                    R = s == 0 ? RV(x, y) : other;
+                    B = s == 0 ? other : RV(x, y);
+                } else {
+                    int rs = 0, rc = 0, bs = 0, bc = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + cr[k][0], ny = y + cr[k][1];
+                        if (site(&ph, ny, nx) == 0) { rs += RV(nx, ny) - (int)G[(size_t)ny * w + nx]; rc++; }
+                    }
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + cr[k][0], ny = y + cr[k][1];
+                        if (site(&ph, ny, nx) == 2) { bs += RV(nx, ny) - (int)G[(size_t)ny * w + nx]; bc++; }
+                    }
+                    R = rc > 0 ? gi + (rs + rc / 2) / rc : gi;
+                    B = bc > 0 ? gi + (bs + bc / 2) / bc : gi;
+                }
+                Gc = gi;
+            } else {
+                if (s != 1) {
+                    float sum = 0.0f;
+                    int cnt = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + dg[k][0], ny = y + dg[k][1];
+                        if (site(&ph, ny, nx) == 2 - s) { sum += (float)RV(nx, ny) - G[(size_t)ny * w + nx]; cnt++; }
+                    }
+                    const int own = (int)((float)RV(x, y) + 0.5f);
+                    const int other = (int)(g + (cnt > 0 ? sum / (float)cnt : 0.0f) + 0.5f);
+                    R = s == 0 ? own : other;
+                    B = s == 0 ? other : own;
+                } else {
+                    float rs = 0.0f, bs = 0.0f;
+                    int rc = 0, bc = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + cr[k][0], ny = y + cr[k][1];
+                        if (site(&ph, ny, nx) == 0) { rs += (float)RV(nx, ny) - G[(size_t)ny * w + nx]; rc++; }
+                    }
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + cr[k][0], ny = y + cr[k][1];
+                        if (site(&ph, ny, nx) == 2) { bs += (float)RV(nx, ny) - G[(size_t)ny * w + nx]; bc++; }
+                    }
+                    R = (int)(g + (rc > 0 ? rs / (float)rc : 0.0f) + 0.5f);
+                    B = (int)(g + (bc > 0 ? bs / (float)bc : 0.0f) + 0.5f);
+                }
+                Gc = (int)(g + 0.5f);
+            }
+            R = R < 0 ? 0 : (R > mx ? mx : R);
+            Gc = Gc < 0 ? 0 : (Gc > mx ? mx : Gc);
+            B = B < 0 ? 0 : (B > mx ? mx : B);
+            put_rgb(rgb, w, x, y, bd, R, Gc, B);
+        }
+#undef RV
+    rgb_borders(rgb, w, h, bd, 5);
+    free(G);
+}
+
 /* ----------------------------------------------------------------- AMAZE */
 /* process_amaze (amaze.cpp:158-200) and its 8-bit core (amaze.cpp:7-156):
  *   G plane starts at -1 everywhere; inside [2, w-2) x [2, h-2) it is the
@@ -958,11 +1069,11 @@ static void amaze(const int *P, uint8_t *rgb, int w, int h, phase_t ph, int bd, 
 }
 
 /* demosaic_cpu (dispatch.cpp:115-128) restricted to SUPER_FAST / HQLI / MG /
- * L7 / AHD / AMAZE / PRISM / RCD:
+ * L7 / DFPD / AHD / AMAZE / PRISM / RCD (every registry entry):
  * validation order and error codes of validate_demosaic_inputs
  * (dispatch.cpp:14-20), then each algorithm's own minimum size
- * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, ahd.cpp:46,
- * amaze.cpp:162, prism.cpp:29, rcd.cpp:11).  Other algorithms
+ * (super_fast.cpp:13, hqli.cpp:11, mg.cpp:12, l7.cpp:69, dfpd.cpp:149,
+ * ahd.cpp:46, amaze.cpp:162, prism.cpp:29, rcd.cpp:11).  Other algorithms
  * return InternalError: they are not restated. */
 int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int algo, int bd, int packed) {
     if (!in || !rgb) return E_NULL;
@@ -974,6 +1085,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     else if (algo == A_SUPER_FAST) min_dim = 2;
     else if (algo == A_L7) min_dim = 8;
     else if (algo == A_AHD || algo == A_AMAZE) min_dim = 6;
+    else if (algo == A_DFPD) min_dim = 12;
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
@@ -985,6 +1097,7 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     else if (algo == A_L7) l7(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_AHD) ahd(P, rgb, w, h, ph, bd);
     else if (algo == A_AMAZE) amaze(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
+    else if (algo == A_DFPD) dfpd(P, rgb, w, h, ph, bd, bd <= 8 && !packed);
     else if (algo == A_HQLI) hqli(P, rgb, w, h, ph, bd, packed);
     else if (algo == A_MG) {
         if (bd <= 8 && !packed) mg_int8(P, rgb, w, h, ph, bd);

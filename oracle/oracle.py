@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libimop_ref.so")
 REF_ISP_PATH = os.path.join(HERE, "_ref", "libimop_ref_isp.so")
 
-SUPER_FAST, HQLI, MG, L7, AHD, AMAZE, RCD, PRISM = 0, 1, 2, 3, 5, 6, 7, 8
+SUPER_FAST, HQLI, MG, L7, DFPD, AHD, AMAZE, RCD, PRISM = 0, 1, 2, 3, 4, 5, 6, 7, 8
 RGGB, BGGR, GRBG, GBRG = 0, 1, 2, 3
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)

@@ -50,7 +50,7 @@ def synthetic_cases():
     rng = np.random.default_rng(20260510)
     k = 0
     for algo, min_dim in ((O.HQLI, 6), (O.MG, 6), (O.PRISM, 10), (O.RCD, 10), (O.SUPER_FAST, 2), (O.L7, 8), (O.AHD, 6),
-                             (O.AMAZE, 6)):
+                             (O.AMAZE, 6), (O.DFPD, 12)):
         for bd, packed in layouts:
             for pattern in range(4):
                 w = int(rng.integers(min_dim, 41))
@@ -61,14 +61,14 @@ def synthetic_cases():
                                   bl=bl_kind, seed=1000 + k))
     # minimum and near-minimum sizes
     for algo, min_dim in ((O.HQLI, 6), (O.MG, 6), (O.PRISM, 10), (O.RCD, 10), (O.SUPER_FAST, 2), (O.L7, 8), (O.AHD, 6),
-                             (O.AMAZE, 6)):
+                             (O.AMAZE, 6), (O.DFPD, 12)):
         for (w, h) in ((min_dim, min_dim), (min_dim + 1, min_dim), (min_dim, min_dim + 3)):
             for bd, packed in ((8, False), (12, True), (10, True)):
                 k += 1
                 cases.append(dict(kind="random", w=w, h=h, bd=bd, packed=packed, pattern=k % 4, algo=algo,
                                   bl="none", seed=1000 + k))
     # seeded scenes at moderate size, all algorithms
-    for algo in (O.HQLI, O.MG, O.PRISM, O.RCD, O.SUPER_FAST, O.L7, O.AHD, O.AMAZE):
+    for algo in (O.HQLI, O.MG, O.PRISM, O.RCD, O.SUPER_FAST, O.L7, O.AHD, O.AMAZE, O.DFPD):
         for bd, packed in ((8, False), (12, True), (10, True)):
             k += 1
             cases.append(dict(kind="scene", w=256, h=192, bd=bd, packed=packed, pattern=0, algo=algo,
@@ -93,6 +93,8 @@ def config_cases():
         dict(kind="scene", w=4024, h=3032, bd=12, packed=True, pattern=0, algo=O.AHD, bl="global",
              bl_value=64.0, seed=2, lines=0),
         dict(kind="scene", w=4024, h=3032, bd=12, packed=True, pattern=0, algo=O.AMAZE, bl="global",
+             bl_value=64.0, seed=2, lines=0),
+        dict(kind="scene", w=4024, h=3032, bd=12, packed=True, pattern=0, algo=O.DFPD, bl="global",
              bl_value=64.0, seed=2, lines=0),
     ]
 
@@ -156,7 +158,7 @@ def main():
     arrays = {}
     for name, (crop, cw, ch, bd, packed) in crops.items():
         arrays[name] = crop
-        for algo in (O.HQLI, O.MG, O.PRISM, O.RCD, O.SUPER_FAST, O.L7, O.AHD, O.AMAZE):
+        for algo in (O.HQLI, O.MG, O.PRISM, O.RCD, O.SUPER_FAST, O.L7, O.AHD, O.AMAZE, O.DFPD):
             for bl in (None, [64.0] * 4):
                 e, rgb = O.ref_demosaic(crop, cw, ch, 0, algo, bd, packed, bl, False)
                 out["cases"].append(dict(kind="crop", name=name, w=cw, h=ch, bd=bd, packed=packed, pattern=0,

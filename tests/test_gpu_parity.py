@@ -75,7 +75,6 @@ def test_error_codes():
     assert imop_amd.demosaic(d_in, d_out, 8, 8, 7, A.MG, 8) == 4
     assert imop_amd.demosaic(d_in, d_out, 5, 8, 0, A.HQLI, 8) == 5
     assert imop_amd.demosaic(d_in, d_out, 9, 12, 0, A.PRISM, 8) == 5
-    assert imop_amd.demosaic(d_in, d_out, 16, 16, 0, A.DFPD, 8) == 6       # no HIP kernel: CPU fallback
     assert imop_amd.demosaic(d_in, d_out, 16, 16, 0, 9, 8) == 6            # not an algorithm
     assert imop_amd.demosaic(d_in, d_out, 16, 16, 0, A.MG, 6, True) == 6   # packed below 8 bits
     small = torch.zeros(10, dtype=torch.uint8, device=DEV)

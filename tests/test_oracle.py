@@ -329,6 +329,26 @@ def test_planted_amaze_fma_is_detected():
     assert diffs > 0
 
 
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("bd,packed", [(8, False), (8, True), (10, True), (12, True), (16, False), (6, False)])
+def test_dfpd_matches_compiled_reference(bd, packed):
+    """DFPD (dfpd.cpp:7-263) against the compiled reference: the integer 8-bit
+    core and the float path, every pattern, sizes from the 12 x 12 minimum,
+    scenes and noise, with and without black level."""
+    rng = np.random.default_rng(bd * 13 + packed)
+    for pattern in range(4):
+        for k, (h, w) in enumerate(((12, 12), (13, 15), (17, 31), (24, 40))):
+            vals = rng.integers(0, 1 << bd, size=(h, w)) if k % 2 else scene.make_scene(h, w, bd, pattern, seed=k)
+            raw = scene.raw_bytes(vals, bd, packed)
+            for bl in (None, [float(rng.integers(0, 40))] * 4):
+                e1, o1 = O.demosaic(raw, w, h, pattern, O.DFPD, bd, packed, bl)
+                e2, o2 = O.ref_demosaic(raw, w, h, pattern, O.DFPD, bd, packed, bl)
+                assert e1 == e2 == 0
+                np.testing.assert_array_equal(o1, o2)
+    z = np.zeros(200, np.uint8)
+    assert O.demosaic(z, 11, 12, 0, O.DFPD, 8)[0] == O.ref_demosaic(z, 11, 12, 0, O.DFPD, 8)[0] == 5
+
+
 def test_l7_closed_forms():
     """A constant frame gives the constant inside [3, w-3) x [3, h-3) and on the
     copied borders, 0 at the four 3 x 3 corners; the weights' centre is 0, so
