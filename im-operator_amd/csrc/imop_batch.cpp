@@ -125,7 +125,7 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
                              nsx <= 65535 && nby <= 65535;
         if (!batched) {
             P.singles.push_back(Single{f, d.algorithm, mode});
-            if (d.algorithm == kPrism || d.algorithm == kRcd || d.algorithm == kAmaze) P.prism_ws = std::max(P.prism_ws, prism_workspace_bytes(d.width, d.height));
+            if (d.algorithm == kPrism || d.algorithm == kRcd) P.prism_ws = std::max(P.prism_ws, prism_workspace_bytes(d.width, d.height));
             continue;
         }
         const auto key = std::make_pair(d.pattern, d.algorithm == kMg ? 1 : 0);

@@ -160,12 +160,7 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
     } else if (algo == kAhd) {
         he = launch_ahd(f, st);
     } else if (algo == kAmaze) {
-        void* ws = prism_ws;
-        if (!ws) {
-            he = get_scratch(amaze_workspace_bytes(f.w, f.h), &ws);
-            if (he != hipSuccess) return kInternal;
-        }
-        he = launch_amaze(f, ws, st);
+        he = launch_amaze(f, st);
     } else if (algo == kPrism || algo == kRcd) {
         // the PRISM / RCD passes index their planes with 32-bit offsets
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;

@@ -34,10 +34,8 @@ hipError_t launch_dfpd(const FrameParams& f, hipStream_t st);
 // AHD for one frame (ahd_kernels.hip).
 hipError_t launch_ahd(const FrameParams& f, hipStream_t st);
 
-// AMAZE for one frame (amaze_kernels.hip); `ws` holds amaze_workspace_bytes()
-// (PRISM's workspace of the same frame is larger).
-size_t amaze_workspace_bytes(int w, int h);
-hipError_t launch_amaze(const FrameParams& f, void* ws, hipStream_t st);
+// AMAZE for one frame (amaze_kernels.hip).
+hipError_t launch_amaze(const FrameParams& f, hipStream_t st);
 
 // PRISM for one frame (prism_kernels.hip).  `ws` holds prism_workspace_bytes().
 size_t prism_workspace_bytes(int w, int h);
