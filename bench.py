@@ -610,6 +610,92 @@ def bench_defect(steps, warmup, with_cpu):
     return out
 
 
+def bench_analyze(steps, warmup, with_cpu):
+    """Raw-file analysis of a buffer already in device memory: imop_hip_analyze_data
+    on a 4024x3032 12-bit frame stored as u16 (24.4 MB: the branch that reads
+    the whole buffer) and on the same frame packed (18.3 MB: a 10000-pixel
+    sample), then imop_hip_guess_pattern on the u16 frame.  Parity against the
+    oracle first.  Each call synchronises its stream and returns its answer to
+    the host, so a call is timed on the host clock; HIP events on the stream
+    give the GPU span."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd = 4024, 3032, 12
+    dev = torch.device("cuda:0")
+    vals = scene.make_scene(h, w, bd, 2, seed=13)
+    bufs = {"u16": scene.raw_bytes(vals, bd, False), "packed": scene.raw_bytes(vals, bd, True)}
+    out = {"workload": "raw-file analysis (analyze_data) of a 4024x3032 12-bit frame stored as u16, 24.4 MB, "
+                       "in device memory"}
+    parity = []
+    for name, buf in bufs.items():
+        d = torch.from_numpy(buf.copy()).to(dev)
+        e, info = imop_amd.analyze_data(d)
+        parity.append(e == 0 and info == O.analyze_data(buf))
+        if name == "u16":
+            e2, pat = imop_amd.guess_pattern(d, w, h, bd, False)
+            parity.append(e2 == 0 and pat == O.guess_pattern(buf, w, h, bd, False))
+        for fn, label in ((lambda: imop_amd.analyze_data(d), "analyze_data"),
+                          (lambda: imop_amd.guess_pattern(d, w, h, bd, False), "guess_pattern")):
+            if label == "guess_pattern" and name != "u16":
+                continue
+            wall, span = [], []
+            for k in range(warmup + steps):
+                torch.cuda.synchronize()
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                t0 = time.perf_counter()
+                a.record()
+                r = fn()
+                b.record()
+                t1 = time.perf_counter()
+                torch.cuda.synchronize()
+                if r[0] != 0:
+                    raise RuntimeError(f"imop_hip_{label} failed")
+                if k >= warmup:
+                    wall.append(t1 - t0)
+                    span.append(a.elapsed_time(b) / 1e3)
+            wall.sort()
+            span.sort()
+            ms = wall[len(wall) // 2] * 1e3
+            gpu_ms = span[len(span) // 2] * 1e3
+            log(f"bench: {label} ({name}, {buf.size} bytes) {ms * 1e3:.1f} us per call (GPU span {gpu_ms * 1e3:.1f} us)")
+            key = label if name == "u16" else f"{label}_{name}"
+            out[key] = {"ms_per_call": round(ms, 5), "gpu_span_ms": round(gpu_ms, 5), "bytes": int(buf.size)}
+            if key == "analyze_data":
+                alg = int(buf.size)  # the buffer read once
+                out.update({"value": round(alg / (ms / 1e3) / 1e9, 1), "unit": "GB/s analysed",
+                            "roofline": {"bound": "hbm", "algorithmic_bytes_per_call": alg,
+                                         "achieved": round(alg / (gpu_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                         "unit": "GB/s", "frac": round(alg / (gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                         "note": "over the GPU span of the call: k_analyze, then the one-workgroup "
+                                                 "k_analyze_final that stores the statistics to pinned host memory"}})
+    out["parity"] = "bit-exact vs oracle" if all(parity) else "MISMATCH"
+    out["steps"], out["warmup"] = steps, warmup
+    if with_cpu:
+        buf = bufs["u16"]
+        fn, ck = (O.ref_analyze_data, "reference") if O.reference_isp_available() else (O.analyze_data, "port")
+        times = []
+        t_end = time.time() + 4.0
+        while len(times) < 3 or (time.time() < t_end and len(times) < 50):
+            t1 = time.perf_counter()
+            fn(buf)
+            times.append(time.perf_counter() - t1)
+        times.sort()
+        med = times[len(times) // 2]
+        out["cpu_baseline"] = {"value": round(buf.size / med / 1e9, 2), "unit": "GB/s analysed", "cores": 1,
+                               "kind": ck, "sample": f"{len(times)} calls on the same u16 buffer, median "
+                                                     f"{med * 1e3:.2f} ms (one thread)"}
+    log(f"bench: analyzer parity {out['parity']}")
+    return out
+
+
 def calibration_copies(mb):
     """4 plain device copies of mb MiB: the known bytes rocprofv3's
     FETCH_SIZE / WRITE_SIZE are scaled against."""
@@ -774,7 +860,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "rcd", "sf", "l7", "ahd", "amaze", "dfpd"),
+    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "analyze", "rcd", "sf", "l7", "ahd", "amaze",
+                                      "dfpd"),
                     default=None,
                     help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
                          "and print its JSON object (for profiling)")
@@ -820,6 +907,11 @@ def main():
     if args.only in ("rcd", "sf", "l7", "ahd", "amaze", "dfpd"):
         names = {"rcd": "rcd10", "sf": "sf12", "l7": "l7_12", "ahd": "ahd12", "amaze": "amaze12", "dfpd": "dfpd12"}
         out = bench_single(names[args.only], args.steps, args.warmup)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "analyze":
+        out = bench_analyze(args.steps, args.warmup, False)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -974,6 +1066,8 @@ def main():
         result["lens_shading"] = bench_lens_shading(100, 5, not args.no_cpu_baseline)
         torch.cuda.empty_cache()
         result["defect_correct"] = bench_defect(30, 3, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        result["analyze"] = bench_analyze(50, 5, not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
 
 

@@ -134,6 +134,31 @@ def lens_shading_params(coef, center=(0.5, 0.5)) -> LensShadingParams:
     return p
 
 
+class DataInfo(ctypes.Structure):
+    """imop_hip_data_info (include/imop_hip.h): imop::DataInfo
+    (include/imop/types.hpp:76-86) with up to 8 possible dimensions."""
+    _fields_ = [
+        ("detected_bit_depth", ctypes.c_int32),
+        ("suggested_width", ctypes.c_int32),
+        ("suggested_height", ctypes.c_int32),
+        ("pixel_count", ctypes.c_int32),
+        ("max_value", ctypes.c_int32),
+        ("min_value", ctypes.c_int32),
+        ("is_likely_16bit", ctypes.c_int32),
+        ("is_packed", ctypes.c_int32),
+        ("n_dimensions", ctypes.c_int32),
+        ("dimensions", (ctypes.c_int32 * 2) * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return dict(detected_bit_depth=self.detected_bit_depth, suggested_width=self.suggested_width,
+                    suggested_height=self.suggested_height, pixel_count=self.pixel_count,
+                    max_value=self.max_value, min_value=self.min_value,
+                    is_likely_16bit=bool(self.is_likely_16bit), is_packed=bool(self.is_packed),
+                    possible_dimensions=[(self.dimensions[k][0], self.dimensions[k][1])
+                                         for k in range(self.n_dimensions)])
+
+
 _lib = None
 
 
@@ -177,6 +202,10 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_lens_shading_poly.restype = i
         L.imop_hip_defect_adaptive.argtypes = [vp, u64, i, i, i, i, ctypes.c_float, vp]
         L.imop_hip_defect_adaptive.restype = i
+        L.imop_hip_analyze_data.argtypes = [vp, u64, ctypes.POINTER(DataInfo), vp]
+        L.imop_hip_analyze_data.restype = i
+        L.imop_hip_guess_pattern.argtypes = [vp, u64, i, i, i, i, ctypes.POINTER(i), vp]
+        L.imop_hip_guess_pattern.restype = i
         L.imop_hip_set_batch_shape.argtypes = [i, i, i, i]
         L.imop_hip_set_batch_shape.restype = i
         L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
@@ -286,6 +315,27 @@ def defect_correct(d_bayer, width: int, height: int, pattern: int, bit_depth: in
     return lib().imop_hip_defect_adaptive(ctypes.c_void_p(d_bayer.data_ptr()),
                                           d_bayer.numel() * d_bayer.element_size(), width, height, int(pattern),
                                           bit_depth, ctypes.c_float(threshold), ctypes.c_void_p(_stream_arg(stream)))
+
+
+def analyze_data(d_data, stream=None) -> tuple[int, dict]:
+    """imop_hip_analyze_data: imop::analyze_data of a raw buffer in device
+    memory (a device tensor, its bytes); (error code, DataInfo as a dict).
+    Synchronises the stream."""
+    info = DataInfo()
+    e = lib().imop_hip_analyze_data(ctypes.c_void_p(d_data.data_ptr()), d_data.numel() * d_data.element_size(),
+                                    ctypes.byref(info), ctypes.c_void_p(_stream_arg(stream)))
+    return e, info.as_dict()
+
+
+def guess_pattern(d_data, width: int, height: int, bit_depth: int, is_packed: bool = False,
+                  stream=None) -> tuple[int, int]:
+    """imop_hip_guess_pattern: imop::guess_pattern of a frame in device memory;
+    (error code, BayerPattern value).  Synchronises the stream."""
+    pat = ctypes.c_int(-1)
+    e = lib().imop_hip_guess_pattern(ctypes.c_void_p(d_data.data_ptr()), d_data.numel() * d_data.element_size(),
+                                     width, height, bit_depth, int(bool(is_packed)), ctypes.byref(pat),
+                                     ctypes.c_void_p(_stream_arg(stream)))
+    return e, pat.value
 
 
 def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,

@@ -272,6 +272,36 @@ int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int heig
 int imop_hip_defect_adaptive(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
                              float threshold, void* stream);
 
+/* --------------------------------------------------------- raw analyzer */
+
+/* imop::DataInfo (include/imop/types.hpp:76-86) with possible_dimensions as
+ * up to 8 (width, height) pairs. */
+typedef struct imop_hip_data_info {
+    int32_t detected_bit_depth;
+    int32_t suggested_width, suggested_height;
+    int32_t pixel_count;
+    int32_t max_value, min_value;
+    int32_t is_likely_16bit, is_packed;
+    int32_t n_dimensions;
+    int32_t dimensions[8][2];
+} imop_hip_data_info;
+
+/* imop::analyze_data (src/analyzer.cpp:151-214) of a raw buffer in device
+ * memory: the statistics its heuristics read (maxima, counts, the whole
+ * buffer's range) are gathered on the GPU, the decisions are the reference's;
+ * same DataInfo.  Synchronises `stream`.  Returns 0, 1 if info is null, 6 on a
+ * HIP failure or a buffer of 2^31 bytes or more. */
+int imop_hip_analyze_data(const void* d_data, uint64_t byte_size, imop_hip_data_info* info, void* stream);
+
+/* imop::guess_pattern (src/analyzer.cpp:216-321) of a frame in device memory
+ * (u8 for bit_depth <= 8, else u16 or packed): *pattern = the BayerPattern
+ * value it returns.  Synchronises `stream`.  Returns 0, 1 if pattern is null,
+ * 3 for a bit depth outside 1..16, 6 if the buffer is smaller than the frame
+ * (depths up to 8 are read one byte per pixel, packed or not) or on a HIP
+ * failure. */
+int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, int height, int bit_depth, int is_packed,
+                           int* pattern, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@
 #
 # A second library, libimop_ref_isp.so, holds the CPU white balance, colour
 # matrix, gamma, polynomial lens-shading and adaptive defect-correction
-# stages.  Their module dispatch files (needed for the input
+# stages, and the raw-data analyzer.  Their module dispatch files (needed for the input
 # validation they call) also reference the CUDA entry points, which a build
 # without CUDA does not define; compiling with hidden visibility and one
 # section per function and linking with --gc-sections drops the unused
@@ -62,6 +62,7 @@ ISP_SRCS=(
     "$REF/src/lens_shading/dispatch.cpp"
     "$REF/src/defect_correct/adaptive.cpp"
     "$REF/src/defect_correct/dispatch.cpp"
+    "$REF/src/analyzer.cpp"
     "$HERE/ref_isp_shim.cpp"
 )
 g++ -std=c++17 -O3 -mavx2 -mfma -fPIC -shared -w -fvisibility=hidden -ffunction-sections -fdata-sections \

@@ -191,3 +191,203 @@ int isp_defect_adaptive(uint8_t *data, int w, int h, int bd, float threshold) {
         }
     return 0;
 }
+
+/* ------------------------------------------------------------ raw analyzer */
+/* match_common_resolution (src/analyzer.cpp:13-25) */
+static size_t an_common(size_t px) {
+    static const size_t sizes[] = {640 * 480,   720 * 480,   720 * 576,   800 * 600,   1024 * 768,  1280 * 720,
+                                   1280 * 800,  1280 * 1024, 1440 * 900,  1920 * 1080, 1920 * 1200, 2048 * 1536,
+                                   2560 * 1440, 2560 * 1600, 3840 * 2160, 4096 * 2160, 4096 * 3072, 5120 * 2880,
+                                   5320 * 4600, 7680 * 4320, 1006 * 758,  4024 * 3032};
+    for (size_t k = 0; k < sizeof sizes / sizeof sizes[0]; ++k)
+        if (px == sizes[k]) return px;
+    return 0;
+}
+
+static unsigned an_u16(const uint8_t *d, size_t i) { return (unsigned)d[2 * i] | ((unsigned)d[2 * i + 1] << 8); }
+
+/* detect_bit_depth (src/analyzer.cpp:27-117) */
+int isp_detect_bit_depth(const uint8_t *d, size_t n) {
+    if (!d || n == 0) return 0;
+    if (an_common(n)) return 8;
+    if (n % 2 == 0 && an_common(n / 2)) { /* guess_bit_from_16bit_data */
+        const size_t c = n / 2 < 100000 ? n / 2 : 100000;
+        unsigned mx = 0;
+        for (size_t i = 0; i < c; ++i) if (an_u16(d, i) > mx) mx = an_u16(d, i);
+        return mx <= 255 ? 8 : mx <= 1023 ? 10 : mx <= 4095 ? 12 : mx <= 16383 ? 14 : 16;
+    }
+    if (n >= 8) { /* looks_like_packed */
+        const size_t c = n / 2 < 10000 ? n / 2 : 10000;
+        int hi = 0, zero = 0;
+        for (size_t i = 0; i < c; ++i) {
+            const unsigned v = an_u16(d, i);
+            if (v >> 8) hi++;
+            if (v == 0) zero++;
+        }
+        if ((double)hi / (double)c > 0.85 && (double)zero / (double)c < 0.01) {
+            static const int ds[4] = {10, 12, 14, 16};
+            for (int k = 0; k < 4; ++k) {
+                const size_t pc = n * 8 / (size_t)ds[k];
+                if (an_common(pc) || an_common(pc + 1)) return ds[k];
+            }
+        }
+    }
+    if (n % 2 == 0 && n >= 4) {
+        const size_t c = n / 2 < 10000 ? n / 2 : 10000;
+        int hi = 0;
+        unsigned mx = 0;
+        for (size_t i = 0; i < c; ++i) {
+            const unsigned v = an_u16(d, i);
+            if (v >> 8) hi++;
+            if (v > mx) mx = v;
+        }
+        if ((double)hi / (double)c > 0.3 && mx > 255) return 16;
+    }
+    return 8;
+}
+
+/* suggest_dimensions (src/analyzer.cpp:119-149): up to 8 pairs into dims */
+int isp_suggest_dimensions(size_t px, int dims[][2], int cap) {
+    static const int cw[] = {640, 720, 800, 1024, 1280, 1440, 1920, 2048, 2560, 3840, 4096, 5120, 7680};
+    static const int ch[] = {480, 576, 600, 720, 768, 800, 1024, 1080, 1200, 1440, 1536, 2048, 2160, 3072, 4320};
+    int n = 0;
+    if (px == 0) return 0;
+    for (size_t a = 0; a < sizeof cw / sizeof cw[0]; ++a)
+        if (px % (size_t)cw[a] == 0) {
+            const int h = (int)(px / (size_t)cw[a]);
+            for (size_t b = 0; b < sizeof ch / sizeof ch[0]; ++b)
+                if (h == ch[b]) { if (n < cap) { dims[n][0] = cw[a]; dims[n][1] = h; } n++; break; }
+        }
+    if (n == 0) {
+        const int s = (int)sqrt((double)px);
+        for (int w = s - 100; w <= s + 100; ++w)
+            if (w > 0 && px % (size_t)w == 0) {
+                const int h = (int)(px / (size_t)w);
+                if (w >= 64 && h >= 64) {
+                    if (n < cap) { dims[n][0] = w; dims[n][1] = h; }
+                    if (++n >= 5) break;
+                }
+            }
+    }
+    return n;
+}
+
+/* analyze_data (src/analyzer.cpp:151-214): out as ref_analyze_data */
+void isp_analyze_data(const uint8_t *d, size_t n, int *out) {
+    for (int k = 0; k < 25; ++k) out[k] = 0;
+    if (!d || n == 0) return;
+    const int bd = isp_detect_bit_depth(d, n);
+    int pc, mn, mx = 0, packed = 0;
+    if (bd > 8) {
+        const size_t p16 = n / 2;
+        if (an_common(p16)) {
+            pc = (int)p16;
+            mn = 65535;
+            for (size_t i = 0; i < p16; ++i) {
+                const int v = (int)an_u16(d, i);
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+        } else {
+            packed = 1;
+            pc = (int)(n * 8 / (size_t)bd);
+            if (!an_common((size_t)pc) && an_common((size_t)pc + 1)) pc++;
+            mn = bd >= 16 ? 65535 : (1 << bd) - 1;
+            const int sample = pc < 10000 ? pc : 10000;
+            for (int i = 0; i < sample; ++i) {
+                const int bo = (i * bd) / 8, bs = (i * bd) % 8;
+                if ((size_t)bo + 1 < n) {
+                    int v = (d[bo] | (d[bo + 1] << 8)) >> bs;
+                    v &= (int)((1u << bd) - 1u);
+                    if (v < mn) mn = v;
+                    if (v > mx) mx = v;
+                }
+            }
+        }
+    } else {
+        pc = (int)n;
+        mn = 255;
+        for (size_t i = 0; i < n; ++i) {
+            if (d[i] < mn) mn = d[i];
+            if (d[i] > mx) mx = d[i];
+        }
+    }
+    out[0] = bd; out[3] = pc; out[4] = mx; out[5] = mn; out[6] = bd > 8; out[7] = packed;
+    int dims[8][2];
+    const int nd = isp_suggest_dimensions((size_t)pc, dims, 8);
+    out[8] = nd < 8 ? nd : 8;
+    for (int k = 0; k < out[8]; ++k) { out[9 + 2 * k] = dims[k][0]; out[10 + 2 * k] = dims[k][1]; }
+    if (nd > 0) { out[1] = dims[0][0]; out[2] = dims[0][1]; }
+}
+
+/* guess_pattern (src/analyzer.cpp:216-321); samples as the demosaic reads
+ * them (8-bit bytes, u16 aligned, packed decoded: oracle's orc_packed_pixel
+ * semantics are reproduced here for the three layouts) */
+static int an_val(const uint8_t *d, int x, int y, int w, int h, int bd, int packed, size_t nb) {
+    if (x < 0 || x >= w || y < 0 || y >= h) return 0;
+    const size_t i = (size_t)y * (size_t)w + (size_t)x;
+    if (bd <= 8) return d[i];
+    if (packed) {
+        extern int orc_packed_pixel(const uint8_t *, size_t, int, size_t);
+        return orc_packed_pixel(d, i, bd, nb);
+    }
+    unsigned v = an_u16(d, i);
+    if (bd < 16 && v > (1u << bd) - 1u) v >>= (16 - bd); /* align_raw_value */
+    return (int)v;
+}
+
+int isp_guess_pattern(const uint8_t *d, int w, int h, int bd, int packed) {
+    if (!d || w < 4 || h < 4) return 0;
+    const size_t nb = packed ? ((size_t)w * (size_t)h * (size_t)bd + 7u) / 8u : (size_t)w * h * (bd <= 8 ? 1u : 2u);
+    const long long npx = (long long)w * h;
+    const int samples = npx < 20000 ? (int)npx : 20000;
+    int dmin = an_val(d, 0, 0, w, h, bd, packed, nb), dmax = dmin;
+    for (int i = 0; i < samples; ++i) {
+        const int idx = (int)((i * 10007) % (int)npx);
+        const int v = an_val(d, idx % w, idx / w, w, h, bd, packed, nb);
+        if (v < dmin) dmin = v;
+        if (v > dmax) dmax = v;
+    }
+    const int range = dmax - dmin > 1 ? dmax - dmin : 1;
+    int step = (int)sqrt((double)w * h / 5000);
+    if (step < 1) step = 1;
+    step = (step / 2) * 2;
+    if (step < 2) step = 2;
+    double conf = range * 0.02;
+    if (conf < 1.0) conf = 1.0;
+    int v1 = 0, v2 = 0;
+    for (int y = 0; y < h - 1; y += step)
+        for (int x = 0; x < w - 1; x += step) {
+            const double tl = an_val(d, x, y, w, h, bd, packed, nb), tr = an_val(d, x + 1, y, w, h, bd, packed, nb);
+            const double bl = an_val(d, x, y + 1, w, h, bd, packed, nb), br = an_val(d, x + 1, y + 1, w, h, bd, packed, nb);
+            const double d1 = fabs(tl - br), d2 = fabs(tr - bl);
+            if (fabs(d1 - d2) < conf) continue;
+            const double hi4 = fmax(fmax(tl, tr), fmax(bl, br)), lo4 = fmin(fmin(tl, tr), fmin(bl, br));
+            if (hi4 - lo4 > range * 0.15) continue;
+            const double hd = fabs(tl - tr) + fabs(bl - br), vd = fabs(tl - bl) + fabs(tr - br);
+            const double mn = hd < vd ? hd : vd, mxd = hd < vd ? vd : hd;
+            if (mxd / (mn > 1.0 ? mn : 1.0) > 4.0) continue;
+            if (d2 < d1) v1++; else v2++;
+        }
+    int gs = (w < h ? w : h) / 256;
+    if (gs < 1) gs = 1;
+    gs |= 1;
+    double s[4] = {0, 0, 0, 0};
+    int c[4] = {0, 0, 0, 0};
+    for (int y = 0; y < h; y += gs)
+        for (int x = 0; x < w; x += gs) {
+            const int k = ((y & 1) == 0 && (x & 1) == 0) ? 0 : ((y & 1) == 1 && (x & 1) == 1) ? 1 : ((y & 1) == 0) ? 2 : 3;
+            s[k] += an_val(d, x, y, w, h, bd, packed, nb);
+            c[k]++;
+        }
+    const int margin = v1 > v2 ? v1 - v2 : v2 - v1;
+    if (margin <= 2 && v1 + v2 < 10) return 0;
+    if (v1 >= v2) {
+        const double ee = c[0] ? s[0] / c[0] : 0, oo = c[1] ? s[1] / c[1] : 0;
+        const double r = ee > 0 ? oo / ee : 0;
+        return r > 1.15 ? 1 : 0; /* BGGR : RGGB */
+    }
+    const double eo = c[2] ? s[2] / c[2] : 0, oe = c[3] ? s[3] / c[3] : 0;
+    const double r = eo > 0 ? oe / eo : 0;
+    return r > 1.15 ? 3 : 2; /* GBRG : GRBG */
+}

@@ -119,6 +119,9 @@ def reference_isp() -> ctypes.CDLL:
         lib.ref_gamma.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [ctypes.c_float]
         lib.ref_lens_shading_poly.argtypes = [_u8p] + [ctypes.c_int] * 4 + [_fp, ctypes.c_float, ctypes.c_float]
         lib.ref_defect_adaptive.argtypes = [_u8p] + [ctypes.c_int] * 4 + [ctypes.c_float]
+        lib.ref_analyze_data.argtypes = [_u8p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+        lib.ref_analyze_data.restype = None
+        lib.ref_guess_pattern.argtypes = [_u8p] + [ctypes.c_int] * 4
         _ref_isp = lib
     return _ref_isp
 
@@ -197,6 +200,46 @@ def ref_defect_adaptive(bayer, bd, threshold=0.3, pattern=0):
     e = reference_isp().ref_defect_adaptive(out.view(np.uint8).ctypes.data_as(_u8p), w, h, pattern, bd,
                                             ctypes.c_float(threshold))
     return e, out
+
+
+def analyze_data(data):
+    """analyze_data (src/analyzer.cpp:151-214), restated: the same dict as ref_analyze_data."""
+    buf = np.ascontiguousarray(data).view(np.uint8).ravel()
+    out = (ctypes.c_int * 25)()
+    lib = oracle()
+    lib.isp_analyze_data.argtypes = [_u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+    lib.isp_analyze_data.restype = None
+    lib.isp_analyze_data(buf.ctypes.data_as(_u8p), buf.size, out)
+    v = list(out)
+    return dict(detected_bit_depth=v[0], suggested_width=v[1], suggested_height=v[2], pixel_count=v[3],
+                max_value=v[4], min_value=v[5], is_likely_16bit=bool(v[6]), is_packed=bool(v[7]),
+                possible_dimensions=[(v[9 + 2 * k], v[10 + 2 * k]) for k in range(v[8])])
+
+
+def guess_pattern(data, w, h, bd, packed=False):
+    """guess_pattern (src/analyzer.cpp:216-321), restated."""
+    buf = np.ascontiguousarray(data).view(np.uint8).ravel()
+    lib = oracle()
+    lib.isp_guess_pattern.argtypes = [_u8p] + [ctypes.c_int] * 4
+    return lib.isp_guess_pattern(buf.ctypes.data_as(_u8p), w, h, bd, int(packed))
+
+
+def ref_analyze_data(data):
+    """imop::analyze_data of the compiled reference (src/analyzer.cpp:151-214):
+    a dict of DataInfo's fields, possible_dimensions as a list of (w, h)."""
+    buf = np.ascontiguousarray(data).view(np.uint8).ravel()
+    out = (ctypes.c_int * 25)()
+    reference_isp().ref_analyze_data(buf.ctypes.data_as(_u8p), buf.size, out)
+    v = list(out)
+    return dict(detected_bit_depth=v[0], suggested_width=v[1], suggested_height=v[2], pixel_count=v[3],
+                max_value=v[4], min_value=v[5], is_likely_16bit=bool(v[6]), is_packed=bool(v[7]),
+                possible_dimensions=[(v[9 + 2 * k], v[10 + 2 * k]) for k in range(v[8])])
+
+
+def ref_guess_pattern(data, w, h, bd, packed=False):
+    """imop::guess_pattern of the compiled reference (src/analyzer.cpp:216-321)."""
+    buf = np.ascontiguousarray(data).view(np.uint8).ravel()
+    return reference_isp().ref_guess_pattern(buf.ctypes.data_as(_u8p), w, h, bd, int(packed))
 
 
 def ref_wb(rgb, bd, gains):

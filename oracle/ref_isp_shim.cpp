@@ -1,6 +1,6 @@
 // ref_isp_shim.cpp -- C entry points over the reference's own CPU white
 // balance, colour matrix, gamma, lens-shading and adaptive defect-correction
-// stages, so tests can call the compiled
+// stages and the raw-data analyzer, so tests can call the compiled
 // reference through ctypes.  TEST INFRASTRUCTURE ONLY: built by
 // oracle/build_ref.sh together with the reference sources under
 // /root/reference into oracle/_ref/libimop_ref_isp.so, used to pin
@@ -9,6 +9,7 @@
 
 #include "ccm/algorithms.hpp"
 #include "defect_correct/algorithms.hpp"
+#include "imop/analyzer.hpp"
 #include "lens_shading/algorithms.hpp"
 #include "tone/algorithms.hpp"
 #include "white_balance/algorithms.hpp"
@@ -60,4 +61,30 @@ REF_EXPORT int ref_defect_adaptive(uint8_t* data, int w, int h, int pattern, int
     p.threshold = threshold;
     return static_cast<int>(
         defect_correct::process_adaptive(data, w, h, static_cast<imop::BayerPattern>(pattern), bd, p));
+}
+
+// imop::analyze_data, src/analyzer.cpp:151-214: out = {detected_bit_depth,
+// suggested_width, suggested_height, pixel_count, max_value, min_value,
+// is_likely_16bit, is_packed, n_dims, then up to 8 (w, h) pairs}.
+REF_EXPORT void ref_analyze_data(const uint8_t* data, uint64_t bytes, int* out) {
+    const imop::DataInfo d = imop::analyze_data(data, static_cast<size_t>(bytes));
+    out[0] = d.detected_bit_depth;
+    out[1] = d.suggested_width;
+    out[2] = d.suggested_height;
+    out[3] = d.pixel_count;
+    out[4] = d.max_value;
+    out[5] = d.min_value;
+    out[6] = d.is_likely_16bit ? 1 : 0;
+    out[7] = d.is_packed ? 1 : 0;
+    const int n = static_cast<int>(d.possible_dimensions.size() < 8 ? d.possible_dimensions.size() : 8);
+    out[8] = n;
+    for (int k = 0; k < n; ++k) {
+        out[9 + 2 * k] = d.possible_dimensions[k].first;
+        out[10 + 2 * k] = d.possible_dimensions[k].second;
+    }
+}
+
+// imop::guess_pattern, src/analyzer.cpp:216-321
+REF_EXPORT int ref_guess_pattern(const uint8_t* data, int w, int h, int bd, int packed) {
+    return static_cast<int>(imop::guess_pattern(data, w, h, bd, packed != 0));
 }

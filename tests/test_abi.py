@@ -80,10 +80,15 @@ def test_no_device_behaves_like_reference_stub():
     ls = imop_amd.lens_shading_params([0.5, 0.0, 0.0] * 4)
     assert lib.imop_hip_lens_shading_poly(None, 0, 8, 8, 0, 8, ctypes.byref(ls), None) == 6
     assert lib.imop_hip_defect_adaptive(None, 0, 8, 8, 0, 8, ctypes.c_float(0.3), None) == 6
+    info = imop_amd.DataInfo()
+    assert lib.imop_hip_analyze_data(None, 0, ctypes.byref(info), None) == 6
+    pat = ctypes.c_int(-1)
+    assert lib.imop_hip_guess_pattern(None, 0, 8, 8, 8, 0, ctypes.byref(pat), None) == 6
 
 
 @pytest.mark.parametrize("ctype,cname", [(imop_amd.FrameDesc, "imop_hip_frame_desc"),
-                                         (imop_amd.RgbEpilogue, "imop_hip_rgb_epilogue")])
+                                         (imop_amd.RgbEpilogue, "imop_hip_rgb_epilogue"),
+                                         (imop_amd.DataInfo, "imop_hip_data_info")])
 def test_struct_layout_matches_header(tmp_path, ctype, cname):
     """The ctypes mirrors of the header's structs have the C compiler's layout."""
     gcc = shutil.which("gcc")

@@ -276,3 +276,91 @@ def test_defect_planted_mistakes_are_detected():
     np.testing.assert_array_equal(_defect_mistaken(img, 8, 0.1), want)
     assert not np.array_equal(_defect_mistaken(img, 8, 0.1, sequential=False), want)
     assert not np.array_equal(_defect_mistaken(img, 8, 0.1, rank=lambda n: (n - 1) // 2), want)
+
+
+# ------------------------------------------------------------ raw analyzer
+import analyzer_cases as AC  # noqa: E402  (tests/analyzer_cases.py)
+
+AN_BUFS = AC.analyze_buffers()
+PAT_FRAMES = AC.pattern_frames()
+
+
+@needs_ref
+@pytest.mark.parametrize("k", range(len(AN_BUFS)), ids=[b[0] for b in AN_BUFS])
+def test_analyze_data_matches_reference(k):
+    buf = AN_BUFS[k][1]()
+    assert O.analyze_data(buf) == O.ref_analyze_data(buf)
+
+
+@needs_ref
+def test_analyze_data_full_sensor_buffers_match_reference():
+    for name, make in AC.analyze_buffers(big=True)[len(AN_BUFS):]:
+        buf = make()
+        assert O.analyze_data(buf) == O.ref_analyze_data(buf), name
+
+
+@needs_ref
+@pytest.mark.parametrize("k", range(len(PAT_FRAMES)), ids=[f[0] for f in PAT_FRAMES])
+def test_guess_pattern_matches_reference(k):
+    _, make, w, h, bd, packed = PAT_FRAMES[k]
+    buf = make()
+    assert O.guess_pattern(buf, w, h, bd, packed) == O.ref_guess_pattern(buf, w, h, bd, packed)
+
+
+def test_analyze_data_closed_forms():
+    rng = np.random.default_rng(3)
+    assert O.analyze_data(np.zeros(0, np.uint8))["detected_bit_depth"] == 0
+    # a known 8-bit size: bytes are pixels
+    b = rng.integers(3, 250, 1280 * 720, dtype=np.uint8)
+    i = O.analyze_data(b)
+    assert (i["detected_bit_depth"], i["pixel_count"], i["min_value"], i["max_value"], i["is_packed"]) == \
+        (8, 1280 * 720, int(b.min()), int(b.max()), False)
+    assert (1280, 720) in i["possible_dimensions"] and (i["suggested_width"], i["suggested_height"]) == \
+        i["possible_dimensions"][0]
+    # u16 storage: the depth comes from the first 100000 values' maximum only,
+    # the range from every value
+    for mx, bd in ((255, 8), (256, 10), (1023, 10), (1024, 12), (4095, 12), (4096, 14), (16383, 14), (16384, 16)):
+        v = rng.integers(0, 200, 640 * 480).astype(np.uint16)
+        v[99999] = mx
+        v[200000] = 65535
+        i = O.analyze_data(v)
+        assert i["detected_bit_depth"] == bd, mx
+        if bd > 8:
+            assert (i["pixel_count"], i["max_value"], i["min_value"], i["is_packed"]) == \
+                (640 * 480, 65535, int(v.min()), False)
+        else:  # judged 8-bit: every byte is a pixel
+            assert i["pixel_count"] == 2 * 640 * 480
+    # an LSB-first stream of a known size: depth, count, and the range of the
+    # first 10000 values as the reference decodes them, from a two-byte window
+    # (a value whose bits reach past the window keeps its low 16 - shift bits)
+    for bd in (10, 12, 14):
+        vals = rng.integers(1, 1 << bd, 1006 * 758)
+        vals[20000] = 0          # beyond the sample: not seen
+        i = O.analyze_data(AC.lsb_stream(vals, bd))
+        assert (i["detected_bit_depth"], i["is_packed"], i["pixel_count"]) == (bd, True, 1006 * 758)
+        shift = (np.arange(10000) * bd) % 8
+        seen = vals[:10000] & ((1 << np.minimum(bd, 16 - shift)) - 1)
+        assert (bd == 14) == (not np.array_equal(seen, vals[:10000]))
+        assert (i["min_value"], i["max_value"]) == (int(seen.min()), int(seen.max()))
+        # one byte short: floor(n * 8 / bd) is one below the known size and is corrected
+        j = O.analyze_data(AC.lsb_stream(vals, bd)[:-1])
+        assert j["pixel_count"] == 1006 * 758 and j["is_packed"]
+
+
+def test_guess_pattern_closed_forms():
+    for pat in range(4):
+        f = AC.ramp_frame(pat)
+        assert O.guess_pattern(f, 256, 256, 16) == pat
+        assert O.guess_pattern(f, 256, 256, 16) == O.ref_guess_pattern(f, 256, 256, 16) if \
+            O.reference_isp_available() else True
+    assert O.guess_pattern(np.full(64 * 48, 77, np.uint8), 64, 48, 8) == 0      # no votes: RGGB
+    assert O.guess_pattern(np.zeros(16, np.uint8), 3, 5, 8) == 0                # too small: RGGB
+
+
+def test_guess_pattern_planted_mistake_is_detected():
+    """Swapping the two families' parity tests (a transposed index) changes
+    the ramp frames' answers: the closed form above pins the indices."""
+    for pat in range(4):
+        f = AC.ramp_frame(pat).T.copy()   # transposing swaps GRBG and GBRG, keeps RGGB and BGGR
+        want = {0: 0, 1: 1, 2: 3, 3: 2}[pat]
+        assert O.guess_pattern(f, 256, 256, 16) == want
