@@ -696,6 +696,104 @@ def bench_analyze(steps, warmup, with_cpu):
     return out
 
 
+def bench_stream(steps, warmup):
+    """Host-buffer demosaic of the mixed batch (48 x 1006x758 + 16 x 4024x3032
+    MG frames, depths {8u, 10p, 12p}, black level {0, 16, 64}, each frame's
+    native RGB element) from and to pageable host memory: (a) one
+    imop_hip_demosaic_host call per frame (a blocking copy in, the launch, a
+    blocking copy out) and (b) imop_hip_demosaic_batch_host (pinned,
+    double-buffered chunks on two streams).  A step is the whole batch, timed
+    on the host clock: PCIe-inclusive throughput.  Pinned copies of the
+    batch's input and output bytes give the link's measured rates."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    MG = imop_amd.DemosaicAlgorithm.MG
+    spec = scene.mixed_batch_spec(seed=4)
+    log(f"bench: host-stream batch of {len(spec)} frames, generating scenes")
+    large = {}
+    raws = []
+    for f in spec:
+        if f["w"] * f["h"] > 4_000_000:
+            key = (f["w"], f["h"], f["bd"])
+            if key not in large:
+                large[key] = scene.raw_bytes(scene.make_scene(f["h"], f["w"], f["bd"], 0, seed=f["bd"]), f["bd"],
+                                             f["packed"])
+            raws.append(large[key].copy())
+        else:
+            raws.append(scene.raw_bytes(scene.make_scene(f["h"], f["w"], f["bd"], 0, seed=f["seed"]), f["bd"],
+                                        f["packed"]))
+    px = sum(f["w"] * f["h"] for f in spec)
+    outs_a = [np.empty(imop_amd.rgb_byte_size(f["w"], f["h"], f["bd"]), np.uint8) for f in spec]
+    outs_b = [np.empty_like(o) for o in outs_a]
+    in_bytes = sum(int(r.size) for r in raws)
+    out_bytes = sum(int(o.size) for o in outs_a)
+
+    def per_frame():
+        for f, r, o in zip(spec, raws, outs_a):
+            e, _ = imop_amd.demosaic_host(r, f["w"], f["h"], 0, MG, f["bd"], f["packed"], [f["bl"]] * 4, out=o)
+            if e != 0:
+                raise RuntimeError("imop_hip_demosaic_host failed")
+
+    descs = [imop_amd.host_frame_desc(r, o, f["w"], f["h"], 0, MG, f["bd"], f["packed"], [f["bl"]] * 4)
+             for f, r, o in zip(spec, raws, outs_b)]
+
+    def streamed():
+        e, bad = imop_amd.demosaic_batch_host(descs)
+        if e != 0:
+            raise RuntimeError(f"imop_hip_demosaic_batch_host failed: {e} at frame {bad}")
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        t = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        t.sort()
+        return t[len(t) // 2] * 1e3
+
+    ms_a = timed(per_frame)
+    ms_b = timed(streamed)
+    parity = all(np.array_equal(a, b) for a, b in zip(outs_a, outs_b))
+    log(f"bench: host stream per-frame {ms_a:.2f} ms, streamed {ms_b:.2f} ms, parity {parity}")
+    # the link: pinned copies of the batch's input and output bytes
+    dev = torch.device("cuda:0")
+    h_in = torch.empty(in_bytes, dtype=torch.uint8).pin_memory()
+    h_out = torch.empty(out_bytes, dtype=torch.uint8).pin_memory()
+    d_in = torch.empty(in_bytes, dtype=torch.uint8, device=dev)
+    d_out = torch.empty(out_bytes, dtype=torch.uint8, device=dev)
+    rates = {}
+    for name, dst, src, nb in (("h2d", d_in, h_in, in_bytes), ("d2h", h_out, d_out, out_bytes)):
+        dst.copy_(src, non_blocking=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            dst.copy_(src, non_blocking=True)
+        torch.cuda.synchronize()
+        rates[name] = nb * 3 / (time.perf_counter() - t0) / 1e9
+    del h_in, h_out, d_in, d_out
+    link_ms = (in_bytes / rates["h2d"] + out_bytes / rates["d2h"]) / 1e6  # bytes / (GB/s) = ns
+    mpx = lambda ms: round(px / (ms / 1e3) / 1e6, 1)
+    return {
+        "workload": "MG demosaic of 48 x 1006x758 + 16 x 4024x3032 frames, depths {8u,10p,12p}, black level "
+                    "{0,16,64}, native RGB element, from and to pageable host memory (PCIe-inclusive)",
+        "frames": len(spec), "pixels_per_step": px, "input_bytes": in_bytes, "output_bytes": out_bytes,
+        "value": mpx(ms_b), "unit": "Mpix/s (host to host)", "ms_per_step": round(ms_b, 3),
+        "per_frame_host_calls_ms": round(ms_a, 3), "per_frame_host_calls_mpix_s": mpx(ms_a),
+        "speedup_vs_per_frame_host_calls": round(ms_a / ms_b, 3),
+        "pinned_copy_gb_s": {k: round(v, 1) for k, v in rates.items()},
+        "pinned_copies_alone_ms": round(link_ms, 3),
+        "frac_of_pinned_copies": round(link_ms / ms_b, 4),
+        "parity": "every frame equals the per-frame host path" if parity else "MISMATCH",
+        "steps": steps, "warmup": warmup,
+    }
+
+
 def calibration_copies(mb):
     """4 plain device copies of mb MiB: the known bytes rocprofv3's
     FETCH_SIZE / WRITE_SIZE are scaled against."""
@@ -860,7 +958,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "chain", "isp", "lens", "defect", "analyze", "rcd", "sf", "l7", "ahd", "amaze",
+    ap.add_argument("--only", choices=("batch", "stream", "chain", "isp", "lens", "defect", "analyze", "rcd", "sf", "l7", "ahd", "amaze",
                                       "dfpd"),
                     default=None,
                     help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
@@ -887,6 +985,10 @@ def main():
         shape = None if args.batch_shape is None else [int(v) for v in args.batch_shape.split(",")]
         out = bench_batch(args.batch_steps, 3, False, shape)
         calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "stream":
+        out = bench_stream(args.batch_steps, 1)
         print(json.dumps(out), flush=True)
         return
     if args.only == "chain":
@@ -1068,6 +1170,8 @@ def main():
         result["defect_correct"] = bench_defect(30, 3, not args.no_cpu_baseline)
         torch.cuda.empty_cache()
         result["analyze"] = bench_analyze(50, 5, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        result["host_stream"] = bench_stream(3, 1)
     print(json.dumps(result), flush=True)
 
 

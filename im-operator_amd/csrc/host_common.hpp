@@ -8,6 +8,8 @@
 
 #include "device_common.hpp"
 
+struct imop_hip_frame_desc;
+
 namespace imop_hip {
 
 // imop::DemosaicError (include/imop/types.hpp:31-39)
@@ -48,5 +50,11 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
 
 // Number of visible devices (probed once).
 int device_count_cached();
+
+// A batch frame's output element size in bytes (out_elem_size 0 -> 1 for
+// depths up to 8, else 2), and imop_hip_demosaic's checks on one batch frame
+// plus its output element and buffer sizes.
+int out_elem_of(const imop_hip_frame_desc& d);
+int check_frame(const imop_hip_frame_desc& d);
 
 }  // namespace imop_hip

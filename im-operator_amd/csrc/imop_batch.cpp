@@ -36,26 +36,6 @@ hipError_t launch_batch(const BatchArgs& a, int pattern, int alg, int rows, int 
     }
 }
 
-namespace {
-
-// Launch shape of k_batch (imop_hip_set_batch_shape).
-std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
-
-int cu_count() {
-    static std::mutex mu;
-    static std::vector<int> cache;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    std::lock_guard<std::mutex> lk(mu);
-    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
-    if (cache[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-        cache[dev] = n;
-    }
-    return cache[dev];
-}
-
 int out_elem_of(const imop_hip_frame_desc& d) {
     return d.out_elem_size == 0 ? (d.bit_depth > 8 ? 2 : 1) : d.out_elem_size;
 }
@@ -75,6 +55,26 @@ int check_frame(const imop_hip_frame_desc& d) {
     if ((d.bit_depth > 8 && !d.is_packed && ((uintptr_t)d.in & 1)) || (elem == 2 && ((uintptr_t)d.out & 1)))
         return kInternal;
     return kOk;
+}
+
+namespace {
+
+// Launch shape of k_batch (imop_hip_set_batch_shape).
+std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
+
+int cu_count() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        cache[dev] = n;
+    }
+    return cache[dev];
 }
 
 struct Group {

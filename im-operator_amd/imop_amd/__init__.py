@@ -188,6 +188,8 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_get_launch_shape.restype = i
         L.imop_hip_demosaic_batch.argtypes = [ctypes.POINTER(FrameDesc), i, vp, ctypes.POINTER(i)]
         L.imop_hip_demosaic_batch.restype = i
+        L.imop_hip_demosaic_batch_host.argtypes = [ctypes.POINTER(FrameDesc), i, ctypes.POINTER(i)]
+        L.imop_hip_demosaic_batch_host.restype = i
         L.imop_hip_batch_plan_create.argtypes = [ctypes.POINTER(FrameDesc), i, ctypes.POINTER(vp), ctypes.POINTER(i)]
         L.imop_hip_batch_plan_create.restype = i
         L.imop_hip_batch_plan_launch.argtypes = [vp, vp]
@@ -410,6 +412,38 @@ def demosaic_batch(descs, stream=None) -> tuple[int, int]:
     failed = ctypes.c_int(-1)
     e = lib().imop_hip_demosaic_batch(arr, len(descs), ctypes.c_void_p(_stream_handle(stream)),
                                       ctypes.byref(failed))
+    return e, failed.value
+
+
+def host_frame_desc(bayer: np.ndarray, out: np.ndarray, width: int, height: int, pattern: int, algorithm: int,
+                    bit_depth: int, is_packed: bool = False, black_level=None, per_channel: bool = False,
+                    out_elem_size: int = 0):
+    """One imop_hip_frame_desc over contiguous host arrays (they must outlive its use)."""
+    d = FrameDesc()
+    if not (bayer.flags.c_contiguous and out.flags.c_contiguous):
+        raise ValueError("host frames must be C-contiguous")
+    d.in_ = bayer.ctypes.data
+    d.in_bytes = bayer.nbytes
+    d.out = out.ctypes.data
+    d.out_bytes = out.nbytes
+    d.width, d.height = width, height
+    d.pattern, d.algorithm, d.bit_depth = int(pattern), int(algorithm), bit_depth
+    d.is_packed = int(bool(is_packed))
+    d.out_elem_size = out_elem_size
+    bl, mode = _bl_args(black_level, per_channel)
+    d.black_level_mode = mode
+    if bl is not None:
+        for k in range(4):
+            d.black_level[k] = float(bl[k])
+    return d
+
+
+def demosaic_batch_host(descs) -> tuple[int, int]:
+    """imop_hip_demosaic_batch_host over FrameDescs of host buffers (pinned,
+    double-buffered staging; synchronous): (DemosaicError, failed frame or -1)."""
+    arr = _desc_array(descs)
+    failed = ctypes.c_int(-1)
+    e = lib().imop_hip_demosaic_batch_host(arr, len(descs), ctypes.byref(failed))
     return e, failed.value
 
 

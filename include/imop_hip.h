@@ -133,6 +133,21 @@ typedef struct imop_hip_frame_desc {
  * and two copies per frame). */
 int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stream, int* failed_frame);
 
+/* The same ragged batch with frames in caller-owned host memory (pageable;
+ * in / out are host pointers sized as in imop_hip_frame_desc, with no
+ * alignment required).  Synchronous: outputs are valid on return.  Frames
+ * are staged in order, in chunks of about 96 MB, through two pinned host and
+ * device slots on two streams, so the input copy of one chunk and the output
+ * copy of the previous one overlap the current chunk's kernels, and the
+ * host-side gathers and scatters (threaded) overlap the GPU.  Every frame is
+ * checked first as imop_hip_demosaic_batch checks it; on the first invalid
+ * frame nothing is copied or written, its index goes to *failed_frame (may
+ * be NULL) and its error code is returned.  Per host thread, the staging
+ * grows to the largest chunk and is kept.  Replaces the double-buffered host
+ * path of imop::demosaic_cuda_batch (src/algorithms/cuda_dispatch.cpp:150-209,
+ * one frame size and depth, one frame in flight per buffer). */
+int imop_hip_demosaic_batch_host(const imop_hip_frame_desc* frames, int n, int* failed_frame);
+
 /* The same batch captured once into a HIP graph, for repeated launches over
  * the same buffers (replaces cuda_graph_capture, src/algorithms/
  * cuda_kernels.cu:1520-1688, which captured one frame size).  The plan owns
