@@ -15,6 +15,7 @@ enum Fmt : int {
     FMT_P12 = 2,   // 2 px / 3 B
     FMT_P10 = 3,   // 4 px / 5 B
     FMT_PGEN = 4,  // LSB-first bitstream, any other depth
+    FMT_G10 = 5,   // GigE Vision 10-bit packing: 2 px / 3 B
 };
 
 // Demosaic modes of the tiled kernels.
@@ -112,6 +113,18 @@ __device__ __forceinline__ int decode_p10(const uint8_t* __restrict__ d, uint64_
     return (int)(((unsigned)d[o + s] << 2) | (((unsigned)d[o + 4] >> (2u * s)) & 3u));
 }
 
+// GigE Vision 10-bit packing (BayerXX10Packed, the layout of the reference's
+// test_data/*_pBayerRG10_Packed.raw): groups of 3 bytes hold two pixels,
+// [P0 bits 9:2][P1 bits 1:0 in bits 5:4 | P0 bits 1:0 in bits 1:0][P1 bits 9:2].
+// A group that runs past the stream decodes as 0.
+__device__ __forceinline__ int decode_g10(const uint8_t* __restrict__ d, uint64_t i, uint64_t nb) {
+    uint64_t o = (i >> 1) * 3u;
+    if (o + 2u >= nb) return 0;
+    unsigned b1 = d[o + 1];
+    if ((i & 1u) == 0) return (int)(((unsigned)d[o] << 2) | (b1 & 3u));
+    return (int)(((unsigned)d[o + 2] << 2) | ((b1 >> 4) & 3u));
+}
+
 // LSB-first bitstream (pixel_utils.hpp:100-117).
 __device__ __forceinline__ int decode_pgen(const uint8_t* __restrict__ d, uint64_t i, uint64_t nb, int bd) {
     uint64_t bit = i * (uint64_t)bd;
@@ -137,6 +150,7 @@ __device__ __forceinline__ int sample_at(const FrameParams& f, int x, int y) {
     case FMT_U16: v = (int)((const uint16_t*)f.in)[i]; break;
     case FMT_P12: v = decode_p12(f.in, i, f.in_bytes); break;
     case FMT_P10: v = decode_p10(f.in, i, f.in_bytes); break;
+    case FMT_G10: v = decode_g10(f.in, i, f.in_bytes); break;
     default: v = decode_pgen(f.in, i, f.in_bytes, f.bd); break;
     }
     if (f.bl_on) {

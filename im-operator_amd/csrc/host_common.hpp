@@ -18,7 +18,7 @@ enum : int { kOk = 0, kNull = 1, kDims = 2, kBitDepth = 3, kPattern = 4, kSmall 
 enum : int { kSuperFast = 0, kHqli = 1, kMg = 2, kL7 = 3, kDfpd = 4, kAhd = 5, kAmaze = 6, kRcd = 7, kPrism = 8 };
 
 // pixel::compute_bayer_byte_size / compute_rgb_byte_size (pixel_utils.hpp:21-34)
-uint64_t bayer_bytes(int w, int h, int bd, bool packed);
+uint64_t bayer_bytes(int w, int h, int bd, int packed);
 uint64_t rgb_bytes(int w, int h, int bd);
 
 // Argument checks in imop::demosaic_cuda's order (cuda_dispatch.cpp:70-90).

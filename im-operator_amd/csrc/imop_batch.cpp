@@ -49,7 +49,7 @@ int check_frame(const imop_hip_frame_desc& d) {
     if (elem != 1 && elem != 2) return kInternal;
     if (elem == 1 && d.bit_depth > 8) return kInternal;
     const uint64_t px = (uint64_t)d.width * (uint64_t)d.height;
-    if (d.in_bytes < bayer_bytes(d.width, d.height, d.bit_depth, d.is_packed != 0) || d.out_bytes < px * 3u * elem)
+    if (d.in_bytes < bayer_bytes(d.width, d.height, d.bit_depth, d.is_packed) || d.out_bytes < px * 3u * elem)
         return kInternal;
     // u16 samples and u16 RGB are accessed as naturally aligned 16-bit words
     if ((d.bit_depth > 8 && !d.is_packed && ((uintptr_t)d.in & 1)) || (elem == 2 && ((uintptr_t)d.out & 1)))

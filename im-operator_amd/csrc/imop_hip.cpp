@@ -31,7 +31,8 @@ int min_dim_of(int algo) {
     }
 }
 
-uint64_t bayer_bytes(int w, int h, int bd, bool packed) {
+uint64_t bayer_bytes(int w, int h, int bd, int packed) {
+    if (packed == 2) return ((uint64_t)w * (uint64_t)h + 1u) / 2u * 3u;  // GigE 10-bit: 3 B per pixel pair
     if (packed) return ((uint64_t)w * (uint64_t)h * (uint64_t)bd + 7u) / 8u;
     return (uint64_t)w * (uint64_t)h * (bd <= 8 ? 1u : 2u);
 }
@@ -49,6 +50,10 @@ int check_args(const void* in, const void* out, int w, int h, int pattern, int a
     // The reference reads packed streams below 8 bits as one byte per pixel
     // (pixel::get_raw), past the end of the stream; leave those to it.
     if (packed && bd < 8) return kInternal;
+    // is_packed: 0 unpacked, 1 the reference's packed layouts, 2 GigE Vision
+    // 10-bit packing (bit depth 10 only)
+    if (packed < 0 || packed > 2) return kInternal;
+    if (packed == 2 && bd != 10) return kBitDepth;
     if (bl_mode < 0 || bl_mode > 2 || (bl_mode != 0 && !bl)) return kInternal;
     return kOk;
 }
@@ -78,7 +83,7 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     // compute_packed_byte_size gives (pixel_utils.hpp:65-66), not by the
     // caller's buffer, which may be larger.
     (void)in_bytes;
-    f.in_bytes = bayer_bytes(w, h, bd, packed != 0);
+    f.in_bytes = bayer_bytes(w, h, bd, packed);
     f.w = w;
     f.h = h;
     f.bd = bd;
@@ -87,6 +92,7 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     f.maxv = (int)((1u << bd) - 1u);
     if (bd <= 8) f.fmt = FMT_U8;
     else if (!packed) f.fmt = FMT_U16;
+    else if (packed == 2) f.fmt = FMT_G10;
     else f.fmt = bd == 12 ? FMT_P12 : bd == 10 ? FMT_P10 : FMT_PGEN;
     f.bl_on = bl_mode != 0;
     if (f.bl_on) bl_by_parity(pattern, bl, bl_mode, f.bl_q);
@@ -183,7 +189,7 @@ int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_
                  int algo, int bd, int packed, const float* bl, int bl_mode, hipStream_t st, const Epilogue* epi) {
     int e = check_args(d_in, d_out, w, h, pattern, algo, bd, packed, bl, bl_mode);
     if (e != kOk) return e;
-    if (in_bytes < bayer_bytes(w, h, bd, packed != 0) || out_bytes < rgb_bytes(w, h, bd)) return kInternal;
+    if (in_bytes < bayer_bytes(w, h, bd, packed) || out_bytes < rgb_bytes(w, h, bd)) return kInternal;
     // u16 samples and u16 RGB are accessed as naturally aligned 16-bit words
     if (bd > 8 && ((!packed && ((uintptr_t)d_in & 1)) || ((uintptr_t)d_out & 1))) return kInternal;
     int mode = -1;
@@ -289,7 +295,7 @@ int imop_hip_demosaic_host(const uint8_t* bayer, uint8_t* rgb, int width, int he
     int e = check_args(bayer, rgb, width, height, pattern, algorithm, bit_depth, is_packed, black_level,
                        black_level_mode);
     if (e != kOk) return e;
-    const uint64_t ib = bayer_bytes(width, height, bit_depth, is_packed != 0);
+    const uint64_t ib = bayer_bytes(width, height, bit_depth, is_packed);
     const uint64_t ob = rgb_bytes(width, height, bit_depth);
     if (!t_staging.ensure(ib, ob)) return kInternal;
     if (hipMemcpy(t_staging.d_in, bayer, ib, hipMemcpyHostToDevice) != hipSuccess) return kInternal;
@@ -332,7 +338,7 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
     if (!data || !offsets) return kNull;
     if (width <= 0 || height <= 0) return kDims;
     if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
-    const uint64_t nb = bayer_bytes(width, height, bit_depth, false);
+    const uint64_t nb = bayer_bytes(width, height, bit_depth, 0);
     if (!t_staging.ensure(nb, 1)) return kInternal;
     if (hipMemcpy(t_staging.d_in, data, nb, hipMemcpyHostToDevice) != hipSuccess) return kInternal;
     int e = imop_hip_black_level(t_staging.d_in, width, height, pattern, algorithm, bit_depth, offsets, nullptr);

@@ -166,7 +166,7 @@ extern "C" int imop_hip_demosaic_batch_host(const imop_hip_frame_desc* frames, i
             if (failed_frame) *failed_frame = i;
             return e;
         }
-        in_n[i] = bayer_bytes(c.width, c.height, c.bit_depth, c.is_packed != 0);
+        in_n[i] = bayer_bytes(c.width, c.height, c.bit_depth, c.is_packed);
         out_n[i] = (size_t)c.width * (size_t)c.height * 3u * (size_t)out_elem_of(c);
     }
     // chunks in frame order

@@ -229,8 +229,23 @@ def cuda_synchronize() -> None:
     lib().imop_hip_synchronize()
 
 
+PACKED_GIGE10 = 2
+"""is_packed value of the GigE Vision 10-bit packing (BayerXX10Packed: 2 px in
+3 bytes, [P0 9:2][P1 1:0 << 4 | P0 1:0][P1 9:2]); True / 1 is the reference's
+packed layouts."""
+
+
+def _packing(is_packed) -> int:
+    """The C ABI's is_packed: 0 unpacked, 1 the reference's packed layouts,
+    2 GigE Vision 10-bit packing."""
+    return PACKED_GIGE10 if (not isinstance(is_packed, bool) and is_packed == PACKED_GIGE10) else int(bool(is_packed))
+
+
 def bayer_byte_size(w: int, h: int, bd: int, packed: bool) -> int:
-    """pixel::compute_bayer_byte_size (include/imop/pixel_utils.hpp:25-28)."""
+    """pixel::compute_bayer_byte_size (include/imop/pixel_utils.hpp:25-28);
+    packed = PACKED_GIGE10: 3 bytes per pixel pair."""
+    if _packing(packed) == PACKED_GIGE10:
+        return (w * h + 1) // 2 * 3
     if packed:
         return (w * h * bd + 7) // 8
     return w * h * (1 if bd <= 8 else 2)
@@ -269,7 +284,7 @@ def demosaic(d_in, d_out, width: int, height: int, pattern: int, algorithm: int,
     out_bytes = d_out.numel() * d_out.element_size()
     return lib().imop_hip_demosaic(ctypes.c_void_p(d_in.data_ptr()), in_bytes, ctypes.c_void_p(d_out.data_ptr()),
                                    out_bytes, width, height, int(pattern), int(algorithm), bit_depth,
-                                   int(bool(is_packed)), blp, mode, ctypes.c_void_p(stream))
+                                   _packing(is_packed), blp, mode, ctypes.c_void_p(stream))
 
 
 def _stream_arg(stream):
@@ -290,7 +305,7 @@ def demosaic_isp(d_in, d_out, width: int, height: int, pattern: int, algorithm: 
     return lib().imop_hip_demosaic_isp(ctypes.c_void_p(d_in.data_ptr()), d_in.numel() * d_in.element_size(),
                                        ctypes.c_void_p(d_out.data_ptr()), d_out.numel() * d_out.element_size(),
                                        width, height, int(pattern), int(algorithm), bit_depth,
-                                       int(bool(is_packed)), blp, mode, ep, ctypes.c_void_p(_stream_arg(stream)))
+                                       _packing(is_packed), blp, mode, ep, ctypes.c_void_p(_stream_arg(stream)))
 
 
 def rgb_epilogue_apply(d_rgb, width: int, height: int, bit_depth: int, epilogue: RgbEpilogue, stream=None) -> int:
@@ -351,7 +366,7 @@ def demosaic_host(bayer: np.ndarray, width: int, height: int, pattern: int, algo
     u8p = ctypes.POINTER(ctypes.c_uint8)
     blp = bl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if bl is not None else None
     e = lib().imop_hip_demosaic_host(raw.ctypes.data_as(u8p), out.ctypes.data_as(u8p), width, height, int(pattern),
-                                     int(algorithm), bit_depth, int(bool(is_packed)), blp, mode)
+                                     int(algorithm), bit_depth, _packing(is_packed), blp, mode)
     rgb = out.view(np.uint16) if bit_depth > 8 else out
     return DemosaicError(e), rgb.reshape(height, width, 3)
 
@@ -389,7 +404,7 @@ def frame_desc(d_in, d_out, width: int, height: int, pattern: int, algorithm: in
     d.out_bytes = d_out.numel() * d_out.element_size()
     d.width, d.height = width, height
     d.pattern, d.algorithm, d.bit_depth = int(pattern), int(algorithm), bit_depth
-    d.is_packed = int(bool(is_packed))
+    d.is_packed = _packing(is_packed)
     d.out_elem_size = out_elem_size
     bl, mode = _bl_args(black_level, per_channel)
     d.black_level_mode = mode
@@ -428,7 +443,7 @@ def host_frame_desc(bayer: np.ndarray, out: np.ndarray, width: int, height: int,
     d.out_bytes = out.nbytes
     d.width, d.height = width, height
     d.pattern, d.algorithm, d.bit_depth = int(pattern), int(algorithm), bit_depth
-    d.is_packed = int(bool(is_packed))
+    d.is_packed = _packing(is_packed)
     d.out_elem_size = out_elem_size
     bl, mode = _bl_args(black_level, per_channel)
     d.black_level_mode = mode

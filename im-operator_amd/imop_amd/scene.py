@@ -149,9 +149,26 @@ def pack(values: np.ndarray, bd: int) -> np.ndarray:
     return np.packbits(bits.reshape(-1, 8), axis=1, bitorder="little").ravel()
 
 
-def raw_bytes(values: np.ndarray, bd: int, packed: bool) -> np.ndarray:
-    """The input buffer for demosaic: packed stream, or u8 / little-endian
-    u16 per pixel."""
+def pack_gige10(values: np.ndarray) -> np.ndarray:
+    """10-bit samples in the GigE Vision 3-byte packing (BayerXX10Packed):
+    per pixel pair [P0 9:2][P1 1:0 << 4 | P0 1:0][P1 9:2]; an odd final pixel
+    gets a zero partner."""
+    v = np.ascontiguousarray(values, dtype=np.uint32).ravel() & 0x3FF
+    if v.size & 1:
+        v = np.concatenate([v, np.zeros(1, np.uint32)])
+    e, o = v[0::2], v[1::2]
+    g = np.empty((e.size, 3), np.uint8)
+    g[:, 0] = e >> 2
+    g[:, 1] = (e & 3) | ((o & 3) << 4)
+    g[:, 2] = o >> 2
+    return g.ravel()
+
+
+def raw_bytes(values: np.ndarray, bd: int, packed) -> np.ndarray:
+    """The input buffer for demosaic: packed stream (packed = 2: the GigE
+    Vision 10-bit packing), or u8 / little-endian u16 per pixel."""
+    if not isinstance(packed, bool) and packed == 2:
+        return pack_gige10(values)
     if packed:
         return pack(values, bd)
     if bd <= 8:

@@ -56,7 +56,12 @@ int imop_hip_synchronize(void);
  * black level.  d_in holds in_bytes bytes, d_out out_bytes bytes; both must
  * be at least the sizes pixel::compute_bayer_byte_size /
  * compute_rgb_byte_size give.  black_level points to 4 host floats (may be
- * NULL when black_level_mode is 0).  Returns after enqueueing. */
+ * NULL when black_level_mode is 0).  is_packed: 0 unpacked, 1 the
+ * reference's packed layouts (pixel_utils.hpp:58-118), 2 the GigE Vision
+ * 10-bit packing (IMOP_HIP_PACKED_GIGE10; bit depth 10 only, else
+ * InvalidBitDepth; 3 bytes per pixel pair), which every entry point taking
+ * is_packed or a frame descriptor accepts.  Returns after enqueueing. */
+#define IMOP_HIP_PACKED_GIGE10 2
 int imop_hip_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
                       int height, int pattern, int algorithm, int bit_depth, int is_packed,
                       const float* black_level, int black_level_mode, void* stream);

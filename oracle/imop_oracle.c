@@ -8,7 +8,8 @@
  *
  * What it restates (reference files are cited per function):
  *   - pixel decode: unpacked u8 / u16 (MSB-aligned fix-up), 12-bit packed
- *     (3 B / 2 px), 10-bit packed (5 B / 4 px), generic LSB-first bitstream
+ *     (3 B / 2 px), 10-bit packed (5 B / 4 px), generic LSB-first bitstream,
+ *     GigE Vision 10-bit packing (3 B / 2 px; is_packed = 2)
  *                                      include/imop/pixel_utils.hpp:58-189
  *   - global / per-channel black level  src/black_level/global.cpp:6-24,
  *                                       src/black_level/per_channel.cpp:6-33
@@ -97,6 +98,7 @@ int orc_max_val(int bd) {
  * pixel_utils.hpp:21-34 */
 size_t orc_packed_bytes(int w, int h, int bd) { return ((size_t)w * (size_t)h * (size_t)bd + 7u) / 8u; }
 size_t orc_bayer_bytes(int w, int h, int bd, int packed) {
+    if (packed == 2) return ((size_t)w * (size_t)h + 1u) / 2u * 3u;
     if (packed) return orc_packed_bytes(w, h, bd);
     return (size_t)w * (size_t)h * (bd <= 8 ? 1u : 2u);
 }
@@ -135,6 +137,18 @@ int orc_packed_pixel(const uint8_t *d, size_t idx, int bd, size_t nbytes) {
     return (int)((v >> sh) & (uint32_t)mx);
 }
 
+/* GigE Vision 10-bit packing (BayerXX10Packed), the layout of the
+ * reference's test_data/<capture>_pBayerRG10_Packed.raw, which its own 10-bit rule
+ * (4 px / 5 B, pixel_utils.hpp:82-98) does not decode: 3-byte groups
+ * [P0 9:2][P1 1:0 << 4 | P0 1:0][P1 9:2], as the GigE Vision pixel format
+ * defines them.  A group past the stream decodes as 0. */
+int orc_gige10_pixel(const uint8_t *d, size_t idx, size_t nbytes) {
+    size_t o = idx / 2u * 3u;
+    if (o + 2u >= nbytes) return 0;
+    if ((idx & 1u) == 0) return (d[o] << 2) | (d[o + 1] & 3);
+    return (d[o + 2] << 2) | ((d[o + 1] >> 4) & 3);
+}
+
 /* align_raw_value, pixel_utils.hpp:129-137: MSB-aligned 9..15-bit samples
  * (value above max) are shifted down. */
 static inline int align_u16(unsigned v, int bd) {
@@ -153,6 +167,9 @@ void orc_decode_plane(const uint8_t *in, int w, int h, int bd, int packed, int *
     const size_t n = (size_t)w * (size_t)h;
     if (bd <= 8) {
         for (size_t i = 0; i < n; ++i) plane[i] = in[i];
+    } else if (packed == 2) {
+        const size_t nb = orc_bayer_bytes(w, h, bd, 2);
+        for (size_t i = 0; i < n; ++i) plane[i] = orc_gige10_pixel(in, i, nb);
     } else if (packed) {
         const size_t nb = orc_packed_bytes(w, h, bd);
         for (size_t i = 0; i < n; ++i) plane[i] = orc_packed_pixel(in, i, bd, nb);
@@ -1089,6 +1106,8 @@ int orc_demosaic(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     else if (algo == A_PRISM || algo == A_RCD) min_dim = 10;
     else return E_INTERNAL;
     if (w < min_dim || h < min_dim) return E_SMALL;
+    if (packed < 0 || packed > 2) return E_INTERNAL;
+    if (packed == 2 && bd != 10) return E_BITDEPTH;  /* GigE packing is defined for 10 bits */
     const phase_t ph = phase_of(pattern);
     int *P = (int *)malloc((size_t)w * h * sizeof(int));
     if (!P) return E_INTERNAL;
@@ -1119,6 +1138,7 @@ int orc_pipeline(const uint8_t *in, uint8_t *rgb, int w, int h, int pattern, int
     if (!in || !rgb) return E_NULL;
     if (w <= 0 || h <= 0) return E_DIMS;
     if (bd < 1 || bd > 16) return E_BITDEPTH;
+    if (packed == 2 && bd != 10) return E_BITDEPTH;
     size_t ub = orc_bayer_bytes(w, h, bd, 0);
     uint8_t *u = (uint8_t *)malloc(ub);
     if (!u) return E_INTERNAL;
