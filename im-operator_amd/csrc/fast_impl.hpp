@@ -72,11 +72,11 @@ struct Raw {
 // (x % 4 == 0 and w % 4 == 0 keep both parts whole numbers of bytes).
 template <int FMT>
 __device__ __forceinline__ uint32_t row_stride(uint32_t w) {
-    return FMT == FMT_U8 ? w : FMT == FMT_U16 ? 2u * w : FMT == FMT_P12 ? w / 2u * 3u : w / 4u * 5u;
+    return FMT == FMT_U8 ? w : FMT == FMT_U16 ? 2u * w : (FMT == FMT_P12 || FMT == FMT_G10) ? w / 2u * 3u : w / 4u * 5u;
 }
 template <int FMT>
 __device__ __forceinline__ uint32_t lane_offset(uint32_t x) {
-    return FMT == FMT_U8 ? x : FMT == FMT_U16 ? 2u * x : FMT == FMT_P12 ? x / 2u * 3u : x / 4u * 5u;
+    return FMT == FMT_U8 ? x : FMT == FMT_U16 ? 2u * x : (FMT == FMT_P12 || FMT == FMT_G10) ? x / 2u * 3u : x / 4u * 5u;
 }
 
 // Edge bands: exact-length loads that never read past the group run, so
@@ -86,7 +86,7 @@ template <int FMT>
 __device__ __forceinline__ Raw load_raw_exact(const FastParams& f, uint32_t off, bool ok) {
     Raw r;
     const uint8_t* p = f.in + (ok ? off : 0u);
-    if (FMT == FMT_P12) {
+    if (FMT == FMT_P12 || FMT == FMT_G10) {
         const uint32_t a = *(const uint16_t*)p, b = *(const uint16_t*)(p + 2), c = *(const uint16_t*)(p + 4);
         r.lo = a | (b << 16);
         r.hi = c;
@@ -122,7 +122,8 @@ __device__ __forceinline__ Raw load_raw(const FastParams& f, uint32_t off, bool 
 template <int FMT>
 __device__ __forceinline__ uint32_t run_offset(uint32_t w, uint32_t x, uint32_t y) {
     const uint32_t i = y * w + x;
-    return FMT == FMT_U8 ? i : FMT == FMT_U16 ? 2u * i : FMT == FMT_P12 ? (i >> 1) * 3u : (i >> 2) * 5u + (i & 2u);
+    return FMT == FMT_U8 ? i : FMT == FMT_U16 ? 2u * i : (FMT == FMT_P12 || FMT == FMT_G10) ? (i >> 1) * 3u
+                                                                            : (i >> 2) * 5u + (i & 2u);
 }
 
 // Bytes off .. off+7 (u8: off .. off+3) from whole aligned words: reads at
@@ -157,6 +158,7 @@ __device__ __forceinline__ uint32_t raw_pixel(const FastParams& f, uint32_t x, u
     if (FMT == FMT_U8) return f.in[i];
     if (FMT == FMT_U16) return ((const uint16_t*)f.in)[i];
     if (FMT == FMT_P12) return (uint32_t)decode_p12(f.in, i, f.in_bytes);
+    if (FMT == FMT_G10) return (uint32_t)decode_g10(f.in, i, f.in_bytes);
     return (uint32_t)decode_p10(f.in, i, f.in_bytes);
 }
 
@@ -181,6 +183,13 @@ __device__ __forceinline__ void raw_values(Raw r, bool sub2, uint32_t v[4]) {
         v[1] = (((r.lo >> 16) & 0xFF) << 4) | (b1 >> 4);
         v[2] = ((r.lo >> 24) << 4) | (b4 & 0xF);
         v[3] = (((r.hi >> 8) & 0xFF) << 4) | (b4 >> 4);
+    } else if (FMT == FMT_G10) {
+        // GigE 10-bit pairs: P0 = b0 << 2 | b1 & 3, P1 = b2 << 2 | (b1 >> 4) & 3
+        const uint32_t b1 = (r.lo >> 8) & 0xFF, b4 = r.hi & 0xFF;
+        v[0] = ((r.lo & 0xFF) << 2) | (b1 & 3);
+        v[1] = (((r.lo >> 16) & 0xFF) << 2) | ((b1 >> 4) & 3);
+        v[2] = ((r.lo >> 24) << 2) | (b4 & 3);
+        v[3] = (((r.hi >> 8) & 0xFF) << 2) | ((b4 >> 4) & 3);
     } else {
         const uint32_t t = r.hi & 0xFF;
         v[0] = ((r.lo & 0xFF) << 2) | (t & 3);
@@ -489,7 +498,7 @@ __device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const 
 template <int FMT, bool CAREFUL, bool ODD>
 __device__ __forceinline__ Raw load_row(const FastParams& f, int y, uint32_t xu, bool lane_ok, uint32_t rs,
                                         uint32_t lo) {
-    constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10;
+    constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10 || FMT == FMT_G10;
     const bool row_ok = !CAREFUL || (y >= 0 && y < f.h);
     if (ODD) {
         // the last row is decoded pixel by pixel (its final group may be

@@ -36,13 +36,14 @@ void get_launch_shape(int* rows, int* wpb, int* xcd_remap) {
 hipError_t launch_fast_u8(const FastParams& f, int mode, hipStream_t st);
 hipError_t launch_fast_u16(const FastParams& f, int mode, hipStream_t st);
 hipError_t launch_fast_p12(const FastParams& f, int mode, hipStream_t st);
+hipError_t launch_fast_g10(const FastParams& f, int mode, hipStream_t st);
 hipError_t launch_fast_p10(const FastParams& f, int mode, hipStream_t st);
 
 bool fast_path_ok(const FrameParams& f) {
     if ((f.w % 4 != 0 && f.w % 4 != 2) || f.w < 8 || f.h < 6) return false;
     if (((uintptr_t)f.in | (uintptr_t)f.out) & 7) return false;
     if ((uint64_t)f.w * f.h * 6u >= (1ull << 31) || f.in_bytes >= (1ull << 31)) return false;  // 32-bit offsets
-    if (f.fmt != FMT_U8 && f.fmt != FMT_U16 && f.fmt != FMT_P12 && f.fmt != FMT_P10) return false;
+    if (f.fmt != FMT_U8 && f.fmt != FMT_U16 && f.fmt != FMT_P12 && f.fmt != FMT_P10 && f.fmt != FMT_G10) return false;
     if (f.bl_on) {
         for (int k = 0; k < 4; ++k) {
             const float o = f.bl_q[k];
@@ -86,6 +87,7 @@ hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epi
     case FMT_U8: return launch_fast_u8(q, mode, st);
     case FMT_U16: return launch_fast_u16(q, mode, st);
     case FMT_P12: return launch_fast_p12(q, mode, st);
+    case FMT_G10: return launch_fast_g10(q, mode, st);
     default: return launch_fast_p10(q, mode, st);
     }
 }

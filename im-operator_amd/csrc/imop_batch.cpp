@@ -121,7 +121,9 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         // k_batch runs MG 8-bit frames on the integer path; a packed 8-bit
         // frame without black level takes the exact path, so it runs alone
         const bool mode_ok = d.algorithm == kHqli || (d.algorithm == kMg && (f.fmt == FMT_U8) == (mode == MODE_MG_INT8));
+        // k_batch decodes u8 / u16 / 12-bit / 10-bit reference-packed runs only
         const bool batched = (d.algorithm == kHqli || d.algorithm == kMg) && mode_ok && fast_path_ok(f) &&
+                             f.fmt != FMT_G10 &&
                              nsx <= 65535 && nby <= 65535;
         if (!batched) {
             P.singles.push_back(Single{f, d.algorithm, mode});
