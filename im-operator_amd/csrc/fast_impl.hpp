@@ -217,6 +217,21 @@ __device__ __forceinline__ void p12_samples(Raw r, int o0, int o1, uint32_t v[4]
     v[0] = x.x; v[1] = y.x; v[2] = x.y; v[3] = y.y;
 }
 
+// GigE 10-bit run (b0..b5 = P0..P3, P0 = b0 << 2 | b1 & 3,
+// P1 = b2 << 2 | (b1 >> 4) & 3, P2 = b3 << 2 | b4 & 3, P3 = b5 << 2 | (b4 >> 4) & 3)
+// the same way: X = (P0, P2) from (b0, b3) and (b1, b4), Y = (P1, P3) from
+// (b2, b5) and (b1, b4) >> 4; samples < 1024, so the halves never carry.
+__device__ __forceinline__ void g10_samples(Raw r, int o0, int o1, uint32_t v[4]) {
+    const uint32_t a = __builtin_amdgcn_perm(r.hi, r.lo, 0x0C030C00u);  // b0, 0, b3, 0
+    const uint32_t b = __builtin_amdgcn_perm(r.hi, r.lo, 0x0C040C01u);  // b1, 0, b4, 0
+    const uint32_t c = __builtin_amdgcn_perm(r.hi, r.lo, 0x0C050C02u);  // b2, 0, b5, 0
+    const u16x2 x = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, (a << 2) | (b & 0x00030003u)),
+                                                  (u16x2){(uint16_t)o0, (uint16_t)o0});
+    const u16x2 y = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, (c << 2) | ((b >> 4) & 0x00030003u)),
+                                                  (u16x2){(uint16_t)o1, (uint16_t)o1});
+    v[0] = x.x; v[1] = y.x; v[2] = x.y; v[3] = y.y;
+}
+
 // Black level (integral offsets o0 / o1 for even / odd columns of the row)
 // or MSB alignment applied to 4 raw values.
 template <int FMT, bool MASK>
@@ -521,8 +536,9 @@ __device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Ra
     if (ODD && CAREFUL && y == f.h - 1) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < f.w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
-    } else if (FMT == FMT_P12) {
-        p12_samples(raw, o0, o1, v);  // black level included
+    } else if (FMT == FMT_P12 || FMT == FMT_G10) {
+        if (FMT == FMT_P12) p12_samples(raw, o0, o1, v);  // black level included
+        else g10_samples(raw, o0, o1, v);
         const bool ok = !CAREFUL || (row_ok && lane_ok);
 #pragma unroll
         for (int k = 0; k < 4; ++k) P[k] = ok ? (int)v[k] : 0;
