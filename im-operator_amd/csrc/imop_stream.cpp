@@ -2,14 +2,15 @@
 // pinned, double-buffered staging (imop_hip_demosaic_batch_host).
 //
 // Frames are grouped in order into chunks of about kChunkBytes (input +
-// output).  Chunk c uses staging slot s = c & 1: a pinned host input and
+// output).  Chunk c uses staging slot s = c % kSlots: a pinned host input and
 // output area, a device input and output area, and stream s.  On stream s:
 //   H2D(c) -> wait for kernel(c-1) -> imop_hip_demosaic_batch(c) -> D2H(c).
 // Kernels of consecutive chunks are serialised through that wait (they share
 // the per-device workspaces), while the copies of neighbouring chunks overlap
-// them: H2D(c + 1) and D2H(c - 1) run during kernel(c).  On the host, the
-// caller's thread scatters chunk c - 2's outputs and gathers chunk c's inputs
-// (threaded memcpy) while the GPU works on chunk c - 1.
+// them.  On the host, the caller's thread scatters chunk c - kSlots's outputs
+// and gathers chunk c's inputs (threaded memcpy) while the GPU works on the
+// chunks in between: with three slots, two chunks' copies stay queued while
+// the host handles a large frame.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +26,7 @@ namespace imop_hip {
 namespace {
 
 constexpr size_t kChunkBytes = (size_t)96 << 20;  // input + output staged per chunk (a larger frame: its own)
+constexpr int kSlots = 3;
 constexpr size_t kParallelCopy = (size_t)4 << 20;  // below this, one thread copies
 
 size_t align256(size_t v) { return (v + 255u) & ~(size_t)255u; }
@@ -76,19 +78,19 @@ void par_copy(const std::vector<Seg>& segs) {
     for (std::thread& t : th) t.join();
 }
 
-// Per-host-thread staging: two slots of pinned host and device memory, two
-// streams (default priority) and their events.  Grows, never shrinks.
+// Per-host-thread staging: kSlots slots of pinned host and device memory,
+// one stream each (default priority) and their events.  Grows, never shrinks.
 struct StreamStaging {
     int device = -1;
-    hipStream_t st[2] = {nullptr, nullptr};
-    hipEvent_t kernel_done[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
-    uint8_t* h[2] = {nullptr, nullptr};
-    uint8_t* d[2] = {nullptr, nullptr};
+    hipStream_t st[kSlots] = {};
+    hipEvent_t kernel_done[kSlots] = {}, done[kSlots] = {};
+    uint8_t* h[kSlots] = {};
+    uint8_t* d[kSlots] = {};
     size_t cap = 0;
 
     ~StreamStaging() { release(); }
     void release() {
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < kSlots; ++s) {
             if (st[s]) (void)hipStreamSynchronize(st[s]);
             if (h[s]) (void)hipHostFree(h[s]);
             if (d[s]) (void)hipFree(d[s]);
@@ -107,7 +109,7 @@ struct StreamStaging {
         if (hipGetDevice(&dev) != hipSuccess) return false;
         if (dev != device) {
             release();
-            for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < kSlots; ++s)
                 if (hipStreamCreateWithFlags(&st[s], hipStreamNonBlocking) != hipSuccess ||
                     hipEventCreateWithFlags(&kernel_done[s], hipEventDisableTiming) != hipSuccess ||
                     hipEventCreateWithFlags(&done[s], hipEventDisableTiming) != hipSuccess) {
@@ -117,13 +119,13 @@ struct StreamStaging {
             device = dev;
         }
         if (cap >= bytes) return true;
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < kSlots; ++s) {
             if (h[s]) (void)hipHostFree(h[s]);
             if (d[s]) (void)hipFree(d[s]);
             h[s] = d[s] = nullptr;
         }
         cap = 0;
-        for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < kSlots; ++s)
             if (hipHostMalloc((void**)&h[s], bytes, hipHostMallocDefault) != hipSuccess ||
                 hipMalloc((void**)&d[s], bytes) != hipSuccess)
                 return false;
@@ -212,24 +214,24 @@ extern "C" int imop_hip_demosaic_batch_host(const imop_hip_frame_desc* frames, i
         par_copy(segs);
     };
     auto fail = [&](int e) {
-        (void)hipStreamSynchronize(S.st[0]);
-        (void)hipStreamSynchronize(S.st[1]);
+        for (int s = 0; s < kSlots; ++s) (void)hipStreamSynchronize(S.st[s]);
         return e;
     };
 
     std::vector<imop_hip_frame_desc> descs;
     const int nc = (int)chunks.size();
     for (int c = 0; c < nc; ++c) {
-        const int s = c & 1;
+        const int s = c % kSlots;
         const Chunk& ch = chunks[c];
-        if (c >= 2) {  // slot s last held chunk c - 2
+        if (c >= kSlots) {  // slot s last held chunk c - kSlots
             if (hipEventSynchronize(S.done[s]) != hipSuccess) return fail(kInternal);
-            scatter(chunks[c - 2], s);
+            scatter(chunks[c - kSlots], s);
         }
         gather(ch, s);
         if (hipMemcpyAsync(S.d[s], S.h[s], ch.in_total, hipMemcpyHostToDevice, S.st[s]) != hipSuccess)
             return fail(kInternal);
-        if (c >= 1 && hipStreamWaitEvent(S.st[s], S.kernel_done[s ^ 1], 0) != hipSuccess) return fail(kInternal);
+        if (c >= 1 && hipStreamWaitEvent(S.st[s], S.kernel_done[(c - 1) % kSlots], 0) != hipSuccess)
+            return fail(kInternal);
         descs.assign(frames + ch.first, frames + ch.first + ch.count);
         for (int k = 0; k < ch.count; ++k) {
             descs[k].in = S.d[s] + ch.at[k].in_off;
@@ -249,8 +251,8 @@ extern "C" int imop_hip_demosaic_batch_host(const imop_hip_frame_desc* frames, i
             hipEventRecord(S.done[s], S.st[s]) != hipSuccess)
             return fail(kInternal);
     }
-    for (int c = std::max(0, nc - 2); c < nc; ++c) {
-        const int s = c & 1;
+    for (int c = std::max(0, nc - kSlots); c < nc; ++c) {
+        const int s = c % kSlots;
         if (hipEventSynchronize(S.done[s]) != hipSuccess) return fail(kInternal);
         scatter(chunks[c], s);
     }

@@ -141,10 +141,10 @@ int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stre
 /* The same ragged batch with frames in caller-owned host memory (pageable;
  * in / out are host pointers sized as in imop_hip_frame_desc, with no
  * alignment required).  Synchronous: outputs are valid on return.  Frames
- * are staged in order, in chunks of about 96 MB, through two pinned host and
- * device slots on two streams, so the input copy of one chunk and the output
- * copy of the previous one overlap the current chunk's kernels, and the
- * host-side gathers and scatters (threaded) overlap the GPU.  Every frame is
+ * are staged in order, in chunks of about 96 MB, through three pinned host
+ * and device slots on three streams, so the copies of neighbouring chunks
+ * overlap the current chunk's kernels, and the host-side gathers and
+ * scatters (threaded) overlap the GPU.  Every frame is
  * checked first as imop_hip_demosaic_batch checks it; on the first invalid
  * frame nothing is copied or written, its index goes to *failed_frame (may
  * be NULL) and its error code is returned.  Per host thread, the staging
