@@ -1,5 +1,5 @@
 // imop_stream.cpp -- demosaic of frames in pageable host memory through
-// pinned, double-buffered staging (imop_hip_demosaic_batch_host).
+// pinned, triple-buffered staging (imop_hip_demosaic_batch_host).
 //
 // Frames are grouped in order into chunks of about kChunkBytes (input +
 // output).  Chunk c uses staging slot s = c % kSlots: a pinned host input and

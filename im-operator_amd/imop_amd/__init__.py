@@ -455,7 +455,7 @@ def host_frame_desc(bayer: np.ndarray, out: np.ndarray, width: int, height: int,
 
 def demosaic_batch_host(descs) -> tuple[int, int]:
     """imop_hip_demosaic_batch_host over FrameDescs of host buffers (pinned,
-    double-buffered staging; synchronous): (DemosaicError, failed frame or -1)."""
+    triple-buffered staging; synchronous): (DemosaicError, failed frame or -1)."""
     arr = _desc_array(descs)
     failed = ctypes.c_int(-1)
     e = lib().imop_hip_demosaic_batch_host(arr, len(descs), ctypes.byref(failed))

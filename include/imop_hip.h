@@ -11,13 +11,15 @@
  *
  * Enumerations are the reference's: pattern = imop::BayerPattern
  * (RGGB 0, BGGR 1, GRBG 2, GBRG 3), algorithm = imop::DemosaicAlgorithm
- * (HQLI 1, MG 2, PRISM 8 have kernels; the other registry entries return 6).
+ * (SUPER_FAST 0, HQLI 1, MG 2, L7 3, DFPD 4, AHD 5, AMAZE 6, RCD 7, PRISM 8;
+ * every one has a kernel).
  *
  * Raw input layouts (include/imop/pixel_utils.hpp:21-189): bit_depth <= 8 one
  * byte per pixel; bit_depth > 8 unpacked is little-endian u16 per pixel;
  * packed 12-bit is 2 px / 3 B, packed 10-bit 4 px / 5 B, other packed depths
  * an LSB-first bitstream, all over the linear pixel index with no row
- * padding.  Output is interleaved RGB, u8 for bit_depth <= 8, else u16.
+ * padding; is_packed = 2 selects the GigE Vision 10-bit packing (2 px / 3 B).
+ * Output is interleaved RGB, u8 for bit_depth <= 8, else u16.
  *
  * Black level (black_level/global.cpp:6-24, per_channel.cpp:6-33):
  * black_level_mode 0 = none; 1 = global (offsets[0] for every site);
