@@ -123,3 +123,25 @@ def test_gige_packing_is_10_bit_only():
                                             int(imop_amd.DemosaicAlgorithm.MG), 10, 3, None, 0,
                                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)) == \
         int(imop_amd.DemosaicError.InternalError)
+
+
+@pytest.mark.parametrize("algo", [imop_amd.DemosaicAlgorithm.MG, imop_amd.DemosaicAlgorithm.HQLI,
+                                  imop_amd.DemosaicAlgorithm.PRISM])
+@pytest.mark.parametrize("flags", [7, 4])
+def test_with_rgb_epilogue(algo, flags):
+    """GigE input through imop_hip_demosaic_isp: the epilogue fused into the
+    register-resident kernel (MG, HQLI) or run as a separate pass (PRISM),
+    against the oracle's demosaic followed by its restated RGB stages."""
+    gains, ccm, gamma = [1.9, 0.8, 2.3], [1.6, -0.4, -0.2, -0.3, 1.5, -0.2, 0.05, -0.55, 1.5], 2.2
+    ep = imop_amd.rgb_epilogue(gains=gains if flags & 1 else None, ccm=ccm if flags & 2 else None,
+                               gamma=gamma if flags & 4 else None)
+    rows = FIX["top"]
+    print(f"gige epilogue {algo.name} flags {flags}", flush=True)
+    d_in = torch.from_numpy(np.ascontiguousarray(rows.ravel()).copy()).to(DEV)
+    d_out = torch.full((W * 64 * 6,), 0xA5, dtype=torch.uint8, device=DEV)
+    e = imop_amd.demosaic_isp(d_in, d_out, W, 64, 0, algo, 10, GIGE, [16.0] * 4, epilogue=ep)
+    torch.cuda.synchronize()
+    eo, rgb = O.demosaic(rows, W, 64, 0, int(algo), 10, GIGE, [16.0] * 4)
+    assert e == eo == 0
+    want = O.isp_epilogue(rgb, 10, flags, gains, ccm, gamma)
+    assert np.array_equal(d_out.cpu().numpy().view(np.uint16).reshape(64, W, 3), want)
