@@ -70,11 +70,18 @@ void par_copy(const std::vector<Seg>& segs) {
     }
     const size_t per = ((total + nt - 1) / nt + 4095u) & ~(size_t)4095u;
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) {
-        const size_t lo = std::min(total, per * t), hi = std::min(total, per * (t + 1));
-        if (lo < hi) th.emplace_back(run, lo, hi);
+    size_t done_to = per;  // ranges [per, done_to) have a thread; the caller copies the rest
+    try {
+        th.reserve(nt - 1);
+        for (int t = 1; t < nt; ++t) {
+            const size_t lo = std::min(total, per * t), hi = std::min(total, per * (t + 1));
+            if (lo < hi) th.emplace_back(run, lo, hi);
+            done_to = std::max(done_to, hi);
+        }
+    } catch (...) {  // no more threads: nothing may escape through the C ABI
     }
     run(0, std::min(total, per));
+    if (done_to < total) run(done_to, total);
     for (std::thread& t : th) t.join();
 }
 
