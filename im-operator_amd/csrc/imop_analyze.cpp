@@ -211,9 +211,10 @@ int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, in
     if (!d_data || width < 4 || height < 4) return kOk;
     if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
     if ((uint64_t)width * (uint64_t)height >= 0x7FFFFFFFull) return kInternal;
-    // depths up to 8 are read one byte per pixel, packed or not (analyzer.cpp:228)
-    const int packed = bit_depth > 8 && is_packed;
-    if (byte_size < bayer_bytes(width, height, bit_depth, packed != 0)) return kInternal;
+    // depths up to 8 are read one byte per pixel, packed or not (analyzer.cpp:228);
+    // is_packed 2 with bit depth 10 is the GigE Vision packing
+    const int packed = bit_depth <= 8 || !is_packed ? 0 : (is_packed == 2 && bit_depth == 10) ? 2 : 1;
+    if (byte_size < bayer_bytes(width, height, bit_depth, packed)) return kInternal;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return kInternal;
     AnalyzeState* S = analyze_state(dev);

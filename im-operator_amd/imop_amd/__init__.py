@@ -350,7 +350,7 @@ def guess_pattern(d_data, width: int, height: int, bit_depth: int, is_packed: bo
     (error code, BayerPattern value).  Synchronises the stream."""
     pat = ctypes.c_int(-1)
     e = lib().imop_hip_guess_pattern(ctypes.c_void_p(d_data.data_ptr()), d_data.numel() * d_data.element_size(),
-                                     width, height, bit_depth, int(bool(is_packed)), ctypes.byref(pat),
+                                     width, height, bit_depth, _packing(is_packed), ctypes.byref(pat),
                                      ctypes.c_void_p(_stream_arg(stream)))
     return e, pat.value
 

@@ -316,7 +316,8 @@ typedef struct imop_hip_data_info {
 int imop_hip_analyze_data(const void* d_data, uint64_t byte_size, imop_hip_data_info* info, void* stream);
 
 /* imop::guess_pattern (src/analyzer.cpp:216-321) of a frame in device memory
- * (u8 for bit_depth <= 8, else u16 or packed): *pattern = the BayerPattern
+ * (u8 for bit_depth <= 8, else u16 or packed; is_packed 2 with bit depth 10
+ * reads the GigE Vision packing): *pattern = the BayerPattern
  * value it returns.  Synchronises `stream`.  Returns 0, 1 if pattern is null,
  * 3 for a bit depth outside 1..16, 6 if the buffer is smaller than the frame
  * (depths up to 8 are read one byte per pixel, packed or not) or on a HIP

@@ -145,3 +145,19 @@ def test_with_rgb_epilogue(algo, flags):
     assert e == eo == 0
     want = O.isp_epilogue(rgb, 10, flags, gains, ccm, gamma)
     assert np.array_equal(d_out.cpu().numpy().view(np.uint16).reshape(64, W, 3), want)
+
+
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+def test_guess_pattern_reads_the_gige_layout(pattern):
+    """imop_hip_guess_pattern with is_packed = 2 sees the same samples as with
+    the frame given unpacked, and answers as the oracle does on that frame."""
+    import analyzer_cases as AC
+    vals = AC.ramp_frame(pattern).astype(np.uint32) >> 6   # 256 x 256, values < 1024
+    h, w = vals.shape
+    g = torch.from_numpy(scene.pack_gige10(vals)).to(DEV)
+    u16 = scene.raw_bytes(vals.astype(np.uint16), 10, False)
+    u = torch.from_numpy(u16.copy()).to(DEV)
+    e1, p1 = imop_amd.guess_pattern(g, w, h, 10, GIGE)
+    e2, p2 = imop_amd.guess_pattern(u, w, h, 10, False)
+    assert e1 == e2 == 0
+    assert p1 == p2 == O.guess_pattern(u16, w, h, 10, False)
