@@ -100,7 +100,9 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
  * of waves (else launch order), bit 1 orders waves strip-fastest (adjacent
  * waves cover adjacent columns of the same rows; else they walk down a
  * strip).  Out-of-range values leave a setting
- * unchanged.  Process-wide; has no counterpart in the reference (its CUDA
+ * unchanged.  Until rows are set, they are 4, or 2 for a frame that would
+ * launch fewer than 4096 waves with 4 (small frames); get reports 4 then.
+ * Process-wide; has no counterpart in the reference (its CUDA
  * launch shape is fixed at 32x16, src/algorithms/cuda_kernels.cu:1496). */
 int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
 int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap);
