@@ -1037,7 +1037,7 @@ def main():
     if args.only in ("rcd", "sf", "l7", "ahd", "amaze", "dfpd", "gige"):
         names = {"rcd": "rcd10", "sf": "sf12", "l7": "l7_12", "ahd": "ahd12", "amaze": "amaze12", "dfpd": "dfpd12",
                  "gige": "mg10g"}
-        out = bench_single(names[args.only], args.steps, args.warmup, graph=args.only == "gige")
+        out = bench_single(names[args.only], args.steps, args.warmup)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -1187,7 +1187,7 @@ def main():
         result["amaze"] = bench_single("amaze12", 50, 5)
         result["dfpd"] = bench_single("dfpd12", 50, 5)
         result["hqli_small"] = bench_single("hqli8", 200, 20, graph=True)
-        result["mg_gige10"] = bench_single("mg10g", 50, 5, graph=True)
+        result["mg_gige10"] = bench_single("mg10g", 50, 5)
     if not args.no_chain and args.workload == "mg12":
         torch.cuda.empty_cache()
         result["chain"] = bench_chain(50, 5, not args.no_cpu_baseline)
