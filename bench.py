@@ -937,18 +937,84 @@ def bench_chain(steps, warmup, with_cpu):
         "warmup": warmup,
     }
     if with_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        raw1 = raw[:1].cpu().numpy()
-        t1 = time.perf_counter()
-        O.chain_fwd64(raw1, 0, 4095.0, g_t, m_t, 2.2, target[:1].cpu().numpy())
-        O.chain_vjp64(raw1, 0, 4095.0, g_t, m_t, 2.2, target=target[:1].cpu().numpy())
-        dt = time.perf_counter() - t1
-        out["cpu_baseline"] = {"value": round(w * h / dt / 1e6, 2), "unit": "Mpix/s (forward+loss and backward)",
-                               "cores": 1, "kind": "port",
-                               "sample": "one 1006x758 frame through the FP64 oracle's forward and VJP "
-                                         "(the reference has no backward), single thread"}
+        out["cpu_baseline"], out["gradient_check"] = chain_cpu_leg(raw, target, g0, m0)
     return out
+
+
+def chain_cpu_leg(raw, target, g0, m0, fd_eps=1e-6):
+    """The FP64 oracle (oracle/chain_oracle.c, single thread) on the whole
+    8-frame batch: its forward + loss and VJP are timed as the CPU baseline,
+    and their results check the GPU backward at the Adam starting point
+    (gains 1.5, CCM = I) -- relative errors ||g - g64|| / ||g64|| per
+    parameter group and for dL/draw over every pixel and away from outputs
+    whose gamma input is below 1.4e-3 (where z^(1/gamma) turns FP32 rounding
+    of M u into large gradient errors); then central differences of the FP64
+    loss (step 1e-6, 24 forwards on worker threads) against the GPU's 12
+    parameter gradients."""
+    import concurrent.futures as cf
+
+    import numpy as np
+    import torch
+
+    from imop_amd import chain as C
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n, h, w = raw.shape
+    gr, gg, gm = C.chain_backward(raw, g0, m0, 0, 4095.0, 2.2, target=target)
+    torch.cuda.synchronize()
+    gr, gg, gm = gr.cpu().numpy(), gg.cpu().numpy().astype(np.float64), gm.cpu().numpy().astype(np.float64)
+    raw_np, t_np = raw.cpu().numpy(), target.cpu().numpy()
+    g64, m64 = g0.cpu().numpy().astype(np.float64), m0.cpu().numpy().astype(np.float64).reshape(3, 3)
+    t1 = time.perf_counter()
+    y64, _ = O.chain_fwd64(raw_np, 0, 4095.0, g64, m64, 2.2, t_np)
+    wr, wg, wm = O.chain_vjp64(raw_np, 0, 4095.0, g64, m64, 2.2, target=t_np)
+    dt = time.perf_counter() - t1
+
+    def rel(a, b):
+        a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    low = (y64 < 0.05).any(axis=-1)
+    near = np.zeros_like(low)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(low, dy, axis=1), dx, axis=2)
+
+    def loss_at(k, sign):
+        g, m = g64.copy(), m64.copy().ravel()
+        if k < 3:
+            g[k] += sign * fd_eps
+        else:
+            m[k - 3] += sign * fd_eps
+        return O.chain_fwd64(raw_np, 0, 4095.0, g, m.reshape(3, 3), 2.2, t_np)[1]
+
+    t2 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        plus = list(ex.map(lambda k: loss_at(k, 1.0), range(12)))
+        minus = list(ex.map(lambda k: loss_at(k, -1.0), range(12)))
+    fd = (np.array(plus) - np.array(minus)) / (2 * fd_eps)
+    fd_s = time.perf_counter() - t2
+    check = {
+        "point": "gains (1.5, 1.5, 1.5), CCM = I, MSE against the rendered target, 8 x 1006x758",
+        "rel_err_vs_fp64_transposed_operator": {
+            "gains": rel(gg, wg), "ccm": rel(gm, wm), "raw_all_pixels": rel(gr, wr),
+            "raw_away_from_z_below_1.4e-3": rel(gr[~near], wr[~near]),
+            "pixels_near_z_below_1.4e-3_pct": round(float(near.mean()) * 100, 4)},
+        "rel_err_vs_central_differences": {"gains": rel(gg, fd[:3]), "ccm": rel(gm, fd[3:]), "step": fd_eps},
+        "tolerance": 1e-4,
+    }
+    log(f"bench: chain gradients vs FP64: gains {check['rel_err_vs_fp64_transposed_operator']['gains']:.2e} "
+        f"ccm {check['rel_err_vs_fp64_transposed_operator']['ccm']:.2e} raw "
+        f"{check['rel_err_vs_fp64_transposed_operator']['raw_all_pixels']:.2e}; vs central differences "
+        f"gains {check['rel_err_vs_central_differences']['gains']:.2e} "
+        f"ccm {check['rel_err_vs_central_differences']['ccm']:.2e} ({fd_s:.1f} s)")
+    base = {"value": round(n * w * h / dt / 1e6, 2), "unit": "Mpix/s (forward+loss and backward)",
+            "cores": 1, "kind": "port",
+            "sample": f"the whole {n} x {w}x{h} batch through the FP64 oracle's forward + loss and VJP "
+                      f"({dt:.2f} s; the reference has no backward), single thread"}
+    return base, check
 
 
 def cpu_baseline_batch(spec, raws, budget_s=10.0):

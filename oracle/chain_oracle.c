@@ -34,6 +34,13 @@
  * the gradient where 0 <= v <= 1 (as torch.clamp does); d/dz z^(1/gamma) is
  * taken as 0 at z = 0.  Loss = mean over every output sample of (y - t)^2.
  *
+ * The forward colour stages are pinned by closed forms (identity
+ * parameters reduce to clamp01(D x); a permutation matrix permutes channels;
+ * gamma 2 maps 0.25 to 0.5; g a > 1 clamps before the matrix) and by the
+ * integer-domain stages of the reference (manual.cpp, linear_3x3.cpp,
+ * gamma.cpp) on inputs where every product is exact; planted mistakes
+ * (orc_chain_set_plant) show each of those checks can fail.
+ *
  * Backward has no counterpart in the reference: it is pinned by central
  * finite differences of this forward, by the adjoint identity of the
  * assembled operator and by the HQLI equivalence of D (tests/test_chain_oracle.py).
@@ -141,21 +148,45 @@ static void csr_free(csr_t* m) {
 
 static double clamp01d(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
 
+/* Planted mistakes, for the tests that show the forward's closed-form checks
+ * can fail (0 = the chain as specified; bits may be combined):
+ *   1  the colour matrix applied transposed (M^T u)
+ *   2  gamma used as the exponent instead of 1 / gamma
+ *   4  white balance without its clamp (u = g .* a, clamped only after the CCM)
+ *   8  the colour matrix before the white balance (u = clamp01(M a), then g) */
+static int g_chain_plant = 0;
+void orc_chain_set_plant(int mask) { g_chain_plant = mask; }
+
 /* Forward of one pixel from its demosaiced a[3]; keeps the intermediates. */
 typedef struct {
     double a[3], b[3], u[3], c[3], z[3], y[3];
 } chain_px;
 
 static void chain_pixel(const double a[3], const double* g, const double* M, double inv_gamma, chain_px* s) {
+    const int tr = g_chain_plant & 1;
+    const double e = (g_chain_plant & 2) ? 1.0 / inv_gamma : inv_gamma;
+    if (g_chain_plant & 8) {
+        double t[3];
+        for (int i = 0; i < 3; ++i) t[i] = clamp01d(M[3 * i] * a[0] + M[3 * i + 1] * a[1] + M[3 * i + 2] * a[2]);
+        for (int k = 0; k < 3; ++k) {
+            s->a[k] = a[k];
+            s->b[k] = s->u[k] = t[k];
+            s->c[k] = g[k] * t[k];
+            s->z[k] = clamp01d(s->c[k]);
+            s->y[k] = s->z[k] > 0.0 ? pow(s->z[k], e) : 0.0;
+        }
+        return;
+    }
     for (int k = 0; k < 3; ++k) {
         s->a[k] = a[k];
         s->b[k] = g[k] * a[k];
-        s->u[k] = clamp01d(s->b[k]);
+        s->u[k] = (g_chain_plant & 4) ? s->b[k] : clamp01d(s->b[k]);
     }
     for (int i = 0; i < 3; ++i) {
-        s->c[i] = M[3 * i] * s->u[0] + M[3 * i + 1] * s->u[1] + M[3 * i + 2] * s->u[2];
+        const double m0 = tr ? M[i] : M[3 * i], m1 = tr ? M[3 + i] : M[3 * i + 1], m2 = tr ? M[6 + i] : M[3 * i + 2];
+        s->c[i] = m0 * s->u[0] + m1 * s->u[1] + m2 * s->u[2];
         s->z[i] = clamp01d(s->c[i]);
-        s->y[i] = s->z[i] > 0.0 ? pow(s->z[i], inv_gamma) : 0.0;
+        s->y[i] = s->z[i] > 0.0 ? pow(s->z[i], e) : 0.0;
     }
 }
 

@@ -69,6 +69,8 @@ def oracle() -> ctypes.CDLL:
                                                                       _dp, _fp, _dp]
         lib.orc_chain_vjp64.argtypes = [_fp] + [ctypes.c_int] * 4 + [ctypes.c_double, _dp, _dp, ctypes.c_double,
                                                                       _fp, _fp, _dp, _dp, _dp]
+        lib.orc_chain_set_plant.argtypes = [ctypes.c_int]
+        lib.orc_chain_set_plant.restype = None
         lib.orc_bilinear_apply64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
         lib.orc_bilinear_apply_t64.argtypes = [_dp] + [ctypes.c_int] * 3 + [_dp]
         lib.isp_wb_manual.argtypes = [_u8p, _u8p] + [ctypes.c_int] * 3 + [_fp]
@@ -366,6 +368,23 @@ def bilinear_apply_t64(v: np.ndarray, pattern: int) -> np.ndarray:
     out = np.zeros(h * w, dtype=np.float64)
     oracle().orc_bilinear_apply_t64(_d(v), w, h, pattern, _d(out))
     return out.reshape(h, w)
+
+
+class chain_plant:
+    """Context manager that plants a mistake in the FP64 chain forward (bits of
+    orc_chain_set_plant: 1 CCM transposed, 2 gamma in place of 1 / gamma,
+    4 white balance unclamped, 8 CCM before white balance)."""
+
+    def __init__(self, mask: int):
+        self.mask = mask
+
+    def __enter__(self):
+        oracle().orc_chain_set_plant(self.mask)
+        return self
+
+    def __exit__(self, *exc):
+        oracle().orc_chain_set_plant(0)
+        return False
 
 
 def chain_fwd64(raw: np.ndarray, pattern: int, max_value: float, gains, ccm, gamma: float, target=None):

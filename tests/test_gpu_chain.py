@@ -148,6 +148,32 @@ def test_gradients_match_central_differences():
     assert rel(got_r, fd_r) <= 1e-4, (got_r, fd_r)
 
 
+def test_parameter_gradients_match_central_differences_full_frame():
+    """One 1006x758 12-bit scene frame: the 12 parameter gradients against
+    central differences (step 1e-6) of the FP64 loss, to 1e-4 relative."""
+    n, w, h, pat = 1, 1006, 758, 0
+    raw, gains, ccm, target = problem(n, w, h, 41, scene_raw=True)
+    g32, m32 = gains.astype(np.float32).astype(np.float64), ccm.astype(np.float32).astype(np.float64)
+    _, gg, gm = C.chain_backward(dev(raw), dev(g32), dev(m32.ravel()), pat, MAXV, GAMMA, target=dev(target),
+                                 want_raw=False)
+    torch.cuda.synchronize()
+
+    def loss(g=g32, m=m32):
+        return O.chain_fwd64(raw, pat, MAXV, g, m, GAMMA, target)[1]
+
+    eps = 1e-6
+    fd = []
+    for k in range(12):
+        d = np.zeros(12)
+        d[k] = eps
+        fd.append((loss(g32 + d[:3], m32 + d[3:].reshape(3, 3)) - loss(g32 - d[:3], m32 - d[3:].reshape(3, 3)))
+                  / (2 * eps))
+    fd = np.array(fd)
+    e_g, e_m = rel(gg.cpu().numpy(), fd[:3]), rel(gm.cpu().numpy().ravel(), fd[3:])
+    print(f"1006x758 vs central differences: gains {e_g:.2e} ccm {e_m:.2e}", flush=True)
+    assert e_g <= 1e-4 and e_m <= 1e-4
+
+
 def test_autograd_functions():
     """ISPChain's backward equals the VJP with the incoming dL/drgb, and
     ISPChainMSE's equals the fused MSE backward."""
