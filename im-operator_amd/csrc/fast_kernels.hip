@@ -11,20 +11,19 @@ namespace imop_hip {
 // Launch shape (set by the autotuner through imop_hip_set_launch_shape), one
 // word so that a launch never sees half of a concurrent change:
 // rows | wpb << 16 | xcd_remap << 24.  xcd_remap bit 0: XCD-contiguous block
-// order; bit 1: strip-fastest wave order.
-static std::atomic<uint32_t> g_shape{4u | (4u << 16) | (3u << 24)};
-// Until rows are set explicitly, a frame that would launch fewer than
+// order; bit 1: strip-fastest wave order.  rows 0 is the automatic band
+// height: 4 rows, except that a frame that would launch fewer than
 // kSmallFrameWaves waves with 4-row bands (under 4 per SIMD) uses 2-row
-// bands: twice the waves to cover the latency of a small frame.
-static std::atomic<bool> g_rows_set{false};
+// bands, twice the waves to cover the latency of a small frame.
+static std::atomic<uint32_t> g_shape{0u | (4u << 16) | (3u << 24)};
 constexpr long kSmallFrameWaves = 4096;
 
 void set_launch_shape(int rows, int wpb, int xcd_remap) {
-    if (rows == 2 || rows == 4 || rows == 8 || rows == 16) g_rows_set.store(true);
+    const bool rows_ok = rows == 0 || rows == 2 || rows == 4 || rows == 8 || rows == 16;
     uint32_t cur = g_shape.load();
     for (;;) {
         int r = (int)(cur & 0xFFFFu), w = (int)((cur >> 16) & 0xFFu), x = (int)(cur >> 24);
-        if (rows == 2 || rows == 4 || rows == 8 || rows == 16) r = rows;
+        if (rows_ok) r = rows;
         if (wpb == 1 || wpb == 2 || wpb == 4) w = wpb;
         if (xcd_remap >= 0 && xcd_remap <= 3) x = xcd_remap;
         const uint32_t next = (uint32_t)r | ((uint32_t)w << 16) | ((uint32_t)x << 24);
@@ -87,9 +86,8 @@ hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epi
     int rows, wpb, xcd;
     get_launch_shape(&rows, &wpb, &xcd);
     const bool with_epi = epi && epi->flags;
-    if (!with_epi && !g_rows_set.load() &&
-        (long)strip_count(f.w, f.out16 != 0) * ((f.h + rows - 1) / rows) < kSmallFrameWaves)
-        rows = 2;
+    if (rows == 0)
+        rows = !with_epi && (long)strip_count(f.w, f.out16 != 0) * ((f.h + 3) / 4) < kSmallFrameWaves ? 2 : 4;
     FastParams q = make_fast(f, with_epi ? 4 : rows, wpb, xcd);
     if (with_epi) q.epi = *epi;
     switch (f.fmt) {

@@ -1,6 +1,6 @@
 // host_common.hpp -- host-side helpers shared by the C ABI translation units:
 // argument checks with the reference's error codes, frame parameter setup,
-// per-device workspaces and the per-frame launch.
+// and the per-frame launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,11 +32,10 @@ void bl_by_parity(int pattern, const float* offs, int mode, float q[4]);
 FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int h, int pattern, int bd, int packed,
                         const float* bl, int bl_mode, int algo, int* mode);
 
-// Per-device PRISM workspace of at least `need` bytes (grows, never shrinks).
-hipError_t get_scratch(size_t need, void** out);
-
-// Enqueue one frame: PRISM (workspace `prism_ws`, or the per-device one when
-// null), the register-resident kernel when it applies, else the LDS-tiled one.
+// Enqueue one frame: PRISM / RCD (planes in `prism_ws`, or, when null, in a
+// stream-ordered allocation of this call released on `st` after its
+// kernels), the register-resident kernel when it applies, else the LDS-tiled
+// one.
 // epi (optional): the RGB epilogue, fused into the register-resident kernel
 // or run as one in-place pass after the demosaic.
 int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st,

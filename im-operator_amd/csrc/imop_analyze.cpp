@@ -210,10 +210,13 @@ int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, in
     *pattern = 0;  // RGGB (analyzer.cpp:218)
     if (!d_data || width < 4 || height < 4) return kOk;
     if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    // is_packed as every other entry point takes it: 0, 1, or 2 (GigE Vision
+    // 10-bit packing, bit depth 10 only)
+    if (is_packed < 0 || is_packed > 2) return kInternal;
+    if (is_packed == 2 && bit_depth != 10) return kBitDepth;
     if ((uint64_t)width * (uint64_t)height >= 0x7FFFFFFFull) return kInternal;
-    // depths up to 8 are read one byte per pixel, packed or not (analyzer.cpp:228);
-    // is_packed 2 with bit depth 10 is the GigE Vision packing
-    const int packed = bit_depth <= 8 || !is_packed ? 0 : (is_packed == 2 && bit_depth == 10) ? 2 : 1;
+    // depths up to 8 are read one byte per pixel, packed or not (analyzer.cpp:228)
+    const int packed = bit_depth <= 8 || !is_packed ? 0 : is_packed;
     if (byte_size < bayer_bytes(width, height, bit_depth, packed)) return kInternal;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return kInternal;

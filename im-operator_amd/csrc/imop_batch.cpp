@@ -222,6 +222,10 @@ int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stre
     int e = prepare(frames, n, P, failed_frame);
     if (e != kOk) return e;
     hipStream_t st = (hipStream_t)stream;
+    // The work lists are copied from this call's host memory, which a graph
+    // replay would read after the call returned: graphs take the plan form.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return kInternal;
     // work lists live in stream-ordered allocations released after the launch
     std::vector<void*> tables(P.groups.size(), nullptr);
     for (size_t i = 0; i < P.groups.size(); ++i) {

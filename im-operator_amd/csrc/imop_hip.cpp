@@ -1,6 +1,6 @@
 // imop_hip.cpp -- C ABI of the gfx950 demosaic path (include/imop_hip.h):
 // argument checks with the reference's error codes, frame parameter setup,
-// device workspaces, the host-buffer (synchronous) variants.
+// stream-ordered PRISM / RCD planes, the host-buffer (synchronous) variants.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -113,36 +113,58 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
     return f;
 }
 
-// Per-device scratch for PRISM planes; grows, never shrinks.
-struct DeviceScratch {
-    void* ptr = nullptr;
-    size_t bytes = 0;
+// PRISM / RCD planes: every call takes its own stream-ordered allocation on
+// the calling stream and releases it on that stream after its kernels.
+//  * Eager calls allocate from a library-owned pool per device whose release
+//    threshold is unlimited, so memory released by one call stays reserved
+//    and the next call of the same frame size maps nothing new.  The pool
+//    reuses a released block on another stream only once that stream is
+//    ordered after the release, so calls on several streams or host threads
+//    never share planes while either is running.
+//  * Under stream capture the allocation and its release become nodes of the
+//    captured graph (hipMallocAsync / hipFreeAsync), so each graph owns its
+//    planes and no later eager call can move or free them.
+struct PlanePools {
+    std::mutex mu;
+    std::vector<hipMemPool_t> pool;
 };
-std::mutex g_scratch_mu;
-std::vector<DeviceScratch> g_scratch;
+PlanePools g_planes;
 
-hipError_t get_scratch(size_t need, void** out) {
+hipError_t plane_pool(hipMemPool_t* out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
-    DeviceScratch& s = g_scratch[dev];
-    if (s.bytes < need) {
-        if (s.ptr) {
-            // Work already queued may still use the old buffer.
-            e = hipDeviceSynchronize();
-            if (e != hipSuccess) return e;
-            (void)hipFree(s.ptr);
-            s.ptr = nullptr;
-            s.bytes = 0;
-        }
-        e = hipMalloc(&s.ptr, need);
+    std::lock_guard<std::mutex> lk(g_planes.mu);
+    if ((int)g_planes.pool.size() <= dev) g_planes.pool.resize(dev + 1, nullptr);
+    if (!g_planes.pool[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t p = nullptr;
+        e = hipMemPoolCreate(&p, &props);
         if (e != hipSuccess) return e;
-        s.bytes = need;
+        uint64_t keep = UINT64_MAX;
+        e = hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(p);
+            return e;
+        }
+        g_planes.pool[dev] = p;
     }
-    *out = s.ptr;
+    *out = g_planes.pool[dev];
     return hipSuccess;
+}
+
+hipError_t planes_alloc(size_t bytes, hipStream_t st, void** out) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(st, &cs);
+    if (e != hipSuccess) return e;
+    if (cs != hipStreamCaptureStatusNone) return hipMallocAsync(out, bytes, st);
+    hipMemPool_t pool = nullptr;
+    e = plane_pool(&pool);
+    if (e != hipSuccess) return e;
+    return hipMallocFromPoolAsync(out, bytes, pool, st);
 }
 
 int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st, const Epilogue* epi) {
@@ -172,10 +194,14 @@ int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipSt
         if ((uint64_t)f.w * (uint64_t)f.h >= (1ull << 31)) return kInternal;
         void* ws = prism_ws;
         if (!ws) {
-            he = get_scratch(prism_workspace_bytes(f.w, f.h), &ws);
+            he = planes_alloc(prism_workspace_bytes(f.w, f.h), st, &ws);
             if (he != hipSuccess) return kInternal;
         }
         he = launch_prism(f, ws, st, algo == kRcd);
+        if (!prism_ws) {
+            const hipError_t fe = hipFreeAsync(ws, st);
+            if (he == hipSuccess) he = fe;
+        }
     } else if (fast_path_ok(f)) {
         he = launch_fast(f, mode, st);
     } else {

@@ -5,9 +5,9 @@
 // output).  Chunk c uses staging slot s = c % kSlots: a pinned host input and
 // output area, a device input and output area, and stream s.  On stream s:
 //   H2D(c) -> wait for kernel(c-1) -> imop_hip_demosaic_batch(c) -> D2H(c).
-// Kernels of consecutive chunks are serialised through that wait (they share
-// the per-device workspaces), while the copies of neighbouring chunks overlap
-// them.  On the host, the caller's thread scatters chunk c - kSlots's outputs
+// Kernels of consecutive chunks are serialised through that wait (one chunk's
+// kernels already fill the device), while the copies of neighbouring chunks
+// overlap them.  On the host, the caller's thread scatters chunk c - kSlots's outputs
 // and gathers chunk c's inputs (threaded memcpy) while the GPU works on the
 // chunks in between: with three slots, two chunks' copies stay queued while
 // the host handles a large frame.

@@ -509,8 +509,9 @@ def get_batch_shape() -> tuple[int, int, int, int]:
     return tuple(x.value for x in v)
 
 
-def set_launch_shape(rows_per_wave: int = 0, waves_per_block: int = 0, xcd_remap: int = -1) -> None:
-    """Launch shape of the register-resident kernel (see imop_hip.h)."""
+def set_launch_shape(rows_per_wave: int = -1, waves_per_block: int = 0, xcd_remap: int = -1) -> None:
+    """Launch shape of the register-resident kernel (imop_hip_set_launch_shape):
+    rows_per_wave 0 = automatic band height, -1 / 0 / -1 = keep a setting."""
     lib().imop_hip_set_launch_shape(rows_per_wave, waves_per_block, xcd_remap)
 
 

@@ -62,7 +62,17 @@ int imop_hip_synchronize(void);
  * reference's packed layouts (pixel_utils.hpp:58-118), 2 the GigE Vision
  * 10-bit packing (IMOP_HIP_PACKED_GIGE10; bit depth 10 only, else
  * InvalidBitDepth; 3 bytes per pixel pair), which every entry point taking
- * is_packed or a frame descriptor accepts.  Returns after enqueueing. */
+ * is_packed or a frame descriptor accepts.  Returns after enqueueing.
+ *
+ * PRISM and RCD keep four FP32 planes (16 bytes per pixel) between their two
+ * passes.  Each call takes them as its own stream-ordered allocation on
+ * `stream` and releases them on `stream` after its kernels, so calls on
+ * different streams or from different host threads never share planes.
+ * Eager calls draw on a library-owned memory pool per device that keeps
+ * released memory reserved (the process holds the largest concurrent need
+ * until it ends); under stream capture the allocation and release become
+ * nodes of the graph, which owns its planes.  The other algorithms use no
+ * device memory beyond d_in and d_out. */
 #define IMOP_HIP_PACKED_GIGE10 2
 int imop_hip_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
                       int height, int pattern, int algorithm, int bit_depth, int is_packed,
@@ -99,10 +109,11 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
  * wave order: bit 0 remaps workgroups so each XCD works on a contiguous run
  * of waves (else launch order), bit 1 orders waves strip-fastest (adjacent
  * waves cover adjacent columns of the same rows; else they walk down a
- * strip).  Out-of-range values leave a setting
- * unchanged.  Until rows are set, they are 4, or 2 for a frame that would
- * launch fewer than 4096 waves with 4 (small frames); get reports 4 then.
- * Process-wide; has no counterpart in the reference (its CUDA
+ * strip).  Out-of-range values (e.g. -1) leave a setting
+ * unchanged.  rows 0 is the automatic band height, the initial setting: 4
+ * rows, or 2 for a frame that would launch fewer than 4096 waves with 4
+ * (small frames; the fused-epilogue form keeps 4); get reports 0 while it is
+ * in effect, so a get / set round trip keeps it.  Process-wide; has no counterpart in the reference (its CUDA
  * launch shape is fixed at 32x16, src/algorithms/cuda_kernels.cu:1496). */
 int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
 int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap);
@@ -137,7 +148,10 @@ typedef struct imop_hip_frame_desc {
  * single-frame launches on the same stream.  Every frame is checked first,
  * as imop_hip_demosaic checks it; on the first invalid frame nothing is
  * enqueued, its index goes to *failed_frame (may be NULL) and its error code
- * is returned.  Replaces imop::demosaic_cuda_batch
+ * is returned.  The work list is copied from host memory of the call, so
+ * the call returns InternalError on a capturing stream (a graph replay would
+ * read that memory after the call); graphs use imop_hip_batch_plan_create.
+ * Replaces imop::demosaic_cuda_batch
  * (src/algorithms/cuda_dispatch.cpp:150-209: one size, one depth, a launch
  * and two copies per frame). */
 int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stream, int* failed_frame);
@@ -321,9 +335,11 @@ int imop_hip_analyze_data(const void* d_data, uint64_t byte_size, imop_hip_data_
  * (u8 for bit_depth <= 8, else u16 or packed; is_packed 2 with bit depth 10
  * reads the GigE Vision packing): *pattern = the BayerPattern
  * value it returns.  Synchronises `stream`.  Returns 0, 1 if pattern is null,
- * 3 for a bit depth outside 1..16, 6 if the buffer is smaller than the frame
+ * 3 for a bit depth outside 1..16 or is_packed 2 with a bit depth other than
+ * 10, 6 for is_packed outside 0..2, if the buffer is smaller than the frame
  * (depths up to 8 are read one byte per pixel, packed or not) or on a HIP
- * failure. */
+ * failure.  A null buffer or a side below 4 gives RGGB and 0, as the
+ * reference returns RGGB there (analyzer.cpp:218). */
 int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, int height, int bit_depth, int is_packed,
                            int* pattern, void* stream);
 

@@ -86,6 +86,28 @@ def test_no_device_behaves_like_reference_stub():
     assert lib.imop_hip_guess_pattern(None, 0, 8, 8, 8, 0, ctypes.byref(pat), None) == 6
 
 
+def test_launch_shape_auto_rows_round_trip():
+    """rows 0 is the automatic band height: get reports it, a get / set round
+    trip keeps it, and -1 / out-of-range values leave a setting unchanged.
+    Only host state; no kernel runs."""
+    saved = imop_amd.get_launch_shape()
+    try:
+        imop_amd.set_launch_shape(0, 4, 3)
+        assert imop_amd.get_launch_shape() == (0, 4, 3)
+        imop_amd.set_launch_shape(*imop_amd.get_launch_shape())
+        assert imop_amd.get_launch_shape() == (0, 4, 3)
+        imop_amd.set_launch_shape(8, 2, 1)
+        assert imop_amd.get_launch_shape() == (8, 2, 1)
+        imop_amd.set_launch_shape(-1, 0, -1)
+        assert imop_amd.get_launch_shape() == (8, 2, 1)
+        imop_amd.set_launch_shape(3, 3, 7)
+        assert imop_amd.get_launch_shape() == (8, 2, 1)
+        imop_amd.set_launch_shape(0)
+        assert imop_amd.get_launch_shape() == (0, 2, 1)
+    finally:
+        imop_amd.set_launch_shape(*saved)
+
+
 @pytest.mark.parametrize("ctype,cname", [(imop_amd.FrameDesc, "imop_hip_frame_desc"),
                                          (imop_amd.RgbEpilogue, "imop_hip_rgb_epilogue"),
                                          (imop_amd.DataInfo, "imop_hip_data_info")])

@@ -92,3 +92,21 @@ def test_guess_pattern_errors():
     # too small or no data: RGGB, as the reference (analyzer.cpp:218)
     assert L.imop_hip_guess_pattern(ptr, d.numel(), 3, 64, 8, 0, ctypes.byref(p), None) == 0 and p.value == 0
     assert L.imop_hip_guess_pattern(None, 0, 64, 64, 8, 0, ctypes.byref(p), None) == 0 and p.value == 0
+
+
+def test_guess_pattern_packing_argument():
+    """is_packed as every entry point takes it: 2 (GigE Vision 10-bit) with
+    another bit depth is InvalidBitDepth, values outside 0..2 InternalError;
+    the valid forms still answer."""
+    L = imop_amd.lib()
+    d = _dev(np.zeros(64 * 64 * 2, np.uint8))
+    p = ctypes.c_int(-1)
+    ptr = ctypes.c_void_p(d.data_ptr())
+    assert L.imop_hip_guess_pattern(ptr, d.numel(), 64, 64, 12, 2, ctypes.byref(p), None) == 3
+    assert L.imop_hip_guess_pattern(ptr, d.numel(), 64, 64, 8, 2, ctypes.byref(p), None) == 3
+    assert L.imop_hip_guess_pattern(ptr, d.numel(), 64, 64, 10, 3, ctypes.byref(p), None) == 6
+    assert L.imop_hip_guess_pattern(ptr, d.numel(), 64, 64, 10, -1, ctypes.byref(p), None) == 6
+    for packed in (0, 1, 2):
+        p.value = -1
+        assert L.imop_hip_guess_pattern(ptr, d.numel(), 64, 64, 10, packed, ctypes.byref(p), None) == 0
+        assert 0 <= p.value <= 3

@@ -148,11 +148,30 @@ def test_gradients_match_central_differences():
     assert rel(got_r, fd_r) <= 1e-4, (got_r, fd_r)
 
 
+def fd_problem(n, w, h, seed):
+    """Scene frames on a pedestal (raw = 0.1 * 4095 + 0.8 * scene) and a
+    colour matrix with positive entries (0.8 I + 0.1 + N(0, 0.02)), so that
+    every gamma input z stays above 0.1.  Near z = 0 the slope of
+    z^(1/gamma) is unbounded, and a pixel within a step of it moves a
+    central difference by ~1e3 / N: on plain scene frames (zeros, and a CCM
+    whose negative entries cross z = 0) the FP64 oracle's own central
+    differences then miss its exact VJP by 1e-3..1e-2 at step 1e-6, while
+    here they agree to ~1e-6."""
+    rng = np.random.default_rng(seed)
+    raw = np.stack([scene.make_scene(h, w, 12, 0, seed=seed * 10 + k) for k in range(n)]).astype(np.float32)
+    raw = (409.5 + 0.8 * raw).astype(np.float32)
+    gains = rng.uniform(1.0, 2.5, 3)
+    ccm = 0.8 * np.eye(3) + 0.1 + rng.normal(0.0, 0.02, (3, 3))
+    target = rng.uniform(0.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    return raw, gains, ccm, target
+
+
 def test_parameter_gradients_match_central_differences_full_frame():
-    """One 1006x758 12-bit scene frame: the 12 parameter gradients against
-    central differences (step 1e-6) of the FP64 loss, to 1e-4 relative."""
+    """One 1006x758 12-bit scene frame (fd_problem): the 12 parameter
+    gradients against central differences (step 1e-6) of the FP64 loss, to
+    1e-4 relative."""
     n, w, h, pat = 1, 1006, 758, 0
-    raw, gains, ccm, target = problem(n, w, h, 41, scene_raw=True)
+    raw, gains, ccm, target = fd_problem(n, w, h, 41)
     g32, m32 = gains.astype(np.float32).astype(np.float64), ccm.astype(np.float32).astype(np.float64)
     _, gg, gm = C.chain_backward(dev(raw), dev(g32), dev(m32.ravel()), pat, MAXV, GAMMA, target=dev(target),
                                  want_raw=False)
