@@ -28,6 +28,17 @@ CXX_SYMBOLS = [
     "_ZN4imop19demosaic_cuda_batchEPKPKhPKPhiiiNS_12BayerPatternENS_17DemosaicAlgorithmEi",
     "_ZN11black_level8has_cudaEv",
     "_ZN11black_level24process_black_level_cudaEPhiiN4imop12BayerPatternENS_19BlackLevelAlgorithmEiRKNS_16BlackLevelParamsE",
+    # src/gpu_dispatch.cpp:341-343, 317-319, 290-292, 455-457, 527-529
+    "_ZN13white_balance8has_cudaEv",
+    "_ZN13white_balance26process_white_balance_cudaEPKhPhiiiifff",
+    "_ZN3ccm8has_cudaEv",
+    "_ZN3ccm16process_ccm_cudaEPKhPhiiiiPKf",
+    "_ZN4tone8has_cudaEv",
+    "_ZN4tone17process_tone_cudaEPKhPhiiiNS_13ToneAlgorithmEiRKNS_10ToneParamsE",
+    "_ZN12lens_shading8has_cudaEv",
+    "_ZN12lens_shading25process_lens_shading_cudaEPhiiN4imop12BayerPatternENS_20LensShadingAlgorithmEiRKNS_17LensShadingParamsE",
+    "_ZN14defect_correct8has_cudaEv",
+    "_ZN14defect_correct27process_defect_correct_cudaEPhiiN4imop12BayerPatternENS_22DefectCorrectAlgorithmEiRKNS_19DefectCorrectParamsE",
 ]
 
 
