@@ -222,6 +222,21 @@ int imop_hip_isp_chain_bwd(const float* d_raw, int n, int width, int height, int
                            const float* d_gains, const float* d_ccm, float gamma, const float* d_grad_out,
                            const float* d_target, float* d_grad_raw, float* d_grad_params, void* stream);
 
+/* imop_hip_isp_chain_bwd without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight training or serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same arguments, checks, kernels and results as imop_hip_isp_chain_bwd; the
+ * per-workgroup partial sums live in a fixed 96 KiB device array
+ * (2048 workgroups x 12 floats) that each GPU holds from the library's load
+ * until the process ends, whose address is looked up on the first call on a
+ * device.  The backward never launches more than 2048 workgroups, so every
+ * size the chain accepts fits; there is no further size limit.  Every call
+ * on one device shares that device's array: calls that overlap on one device
+ * (different streams, or host threads without ordering) race. */
+int imop_hip_isp_chain_bwd_noalloc(const float* d_raw, int n, int width, int height, int pattern, float max_value,
+                                   const float* d_gains, const float* d_ccm, float gamma, const float* d_grad_out,
+                                   const float* d_target, float* d_grad_raw, float* d_grad_params, void* stream);
+
 /* The chain's demosaic operator alone, FP32, one frame: out[h][w][3] = D x
  * and out[h][w] = D^T v (for checks against the transposed operator). */
 int imop_hip_bilinear_fwd(const float* d_x, int width, int height, int pattern, float* d_out, void* stream);
