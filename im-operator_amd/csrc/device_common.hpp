@@ -83,9 +83,10 @@ struct FastParams {
     int32_t bl_on;
     int32_t bl_i[4];      // integral black level by (y & 1) * 2 + (x & 1)
     int32_t th;           // rows per wave (even)
-    int32_t nsx, nby;     // strips (strip_count) and bands
+    int32_t nsx, nby;     // strips (strip_count) and bands (single-frame kernel: groups of wpb bands)
     int32_t nwaves;
     int32_t wpb;          // waves per workgroup
+    uint64_t place_mul;   // single-frame kernel: ceil(2^40 / divisor) of the group placement
     int32_t xcd_remap;    // give each XCD a contiguous run of waves
     int32_t aligned;      // aligned strips for u16 output (strip_stride)
     Epilogue epi;         // RGB stages applied before the stores (k_fast<.., EPI = true>)

@@ -32,6 +32,8 @@
 // black level absent or integral per site.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "epilogue.hpp"
 #include "plain.hpp"
@@ -424,6 +426,32 @@ __device__ __forceinline__ void store_row_staged(const FastParams& f, uint32_t* 
     }
 }
 
+// store_row_staged for u16 RGB rows whose window is whole 16-byte chunks
+// (interior rows of aligned strips: 1344 bytes): lanes 0 and 63 stage their
+// unused pixels past the 1488 staged bytes instead of branching around the
+// LDS writes, and both chunk reads are unconditional (the second may read
+// past the staged row, inside the caller's LDS).
+__device__ __forceinline__ void store_row_u16_whole(const FastParams& f, uint32_t* stage, int lane, uint32_t row_px,
+                                                    int c0, int c1, const int r[4], const int g[4],
+                                                    const int b[4]) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(6u * (uint32_t)c0);
+    const uint32_t nbytes = __builtin_amdgcn_readfirstlane(6u * (uint32_t)(c1 - c0));
+    uint8_t* base = f.out + 6ull * (row_px + (uint32_t)c0);
+    const uint32_t slot = (uint32_t)(lane - 1) < 62u ? 6u * (uint32_t)(lane - 1) : 372u;
+    uint2* p = (uint2*)(stage + slot);
+    p[0] = make_uint2((uint32_t)r[0] | ((uint32_t)g[0] << 16), (uint32_t)b[0] | ((uint32_t)r[1] << 16));
+    p[1] = make_uint2((uint32_t)g[1] | ((uint32_t)b[1] << 16), (uint32_t)r[2] | ((uint32_t)g[2] << 16));
+    p[2] = make_uint2((uint32_t)b[2] | ((uint32_t)r[3] << 16), (uint32_t)g[3] | ((uint32_t)b[3] << 16));
+    const uint32_t o0 = 16u * (uint32_t)lane, o1 = 1024u + o0;
+    lds_order();
+    const uint4 v0 = ((const uint4*)stage)[(b0 >> 4) + (uint32_t)lane];
+    const uint4 v1 = ((const uint4*)stage)[(b0 >> 4) + 64u + (uint32_t)lane];
+    lds_order();
+    if (o0 < nbytes) __builtin_nontemporal_store((u32x4){v0.x, v0.y, v0.z, v0.w}, (u32x4*)(base + o0));
+    if (o1 < nbytes) __builtin_nontemporal_store((u32x4){v1.x, v1.y, v1.z, v1.w}, (u32x4*)(base + o1));
+}
+
 // MG colour difference of the two R/B sites a lane owns in one row.
 // P points at 5 window rows centred on the row (P[0] two rows up).
 // Halo columns come from the neighbouring lanes.
@@ -510,30 +538,33 @@ __device__ __forceinline__ void mg_px(int a, int k, int c, const int* Dm, const 
 // Raw bytes of the lane run in row y.  Interior bands: every row and lane
 // is inside the frame, so no masking; their 8-byte windows end
 // before the stream does because the final row is never in their window.
-template <int FMT, bool CAREFUL, bool ODD>
+// VRIM: the band's rows reach the top or bottom rim (rows outside the frame,
+// the stream's final row); CAREFUL without VRIM: only lanes outside the
+// frame's columns, with every row inside and followed by another.
+template <int FMT, bool CAREFUL, bool ODD, bool VRIM = CAREFUL>
 __device__ __forceinline__ Raw load_row(const FastParams& f, int y, uint32_t xu, bool lane_ok, uint32_t rs,
                                         uint32_t lo) {
     constexpr bool packed = FMT == FMT_P12 || FMT == FMT_P10 || FMT == FMT_G10;
-    const bool row_ok = !CAREFUL || (y >= 0 && y < f.h);
+    const bool row_ok = !VRIM || (y >= 0 && y < f.h);
     if (ODD) {
         // the last row is decoded pixel by pixel (its final group may be
         // cut short); every other row has >= 11 stream bytes past the run
-        const bool ok = row_ok && lane_ok && !(CAREFUL && y == f.h - 1);
+        const bool ok = row_ok && lane_ok && !(VRIM && y == f.h - 1);
         return load_raw_any<FMT>(f, ok ? run_offset<FMT>((uint32_t)f.w, xu, (uint32_t)y) : 0u);
     }
     const uint32_t off = (row_ok ? (uint32_t)y : 0u) * rs + lo;
-    if (CAREFUL && packed) return load_raw_exact<FMT>(f, off, row_ok && lane_ok);
+    if (VRIM && packed) return load_raw_exact<FMT>(f, off, row_ok && lane_ok);
     return load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
 }
 
 // Decoded, black-levelled samples of row y (parity YODD) from its raw bytes.
-template <int FMT, bool CAREFUL, bool ODD, int YODD>
+template <int FMT, bool CAREFUL, bool ODD, int YODD, bool VRIM = CAREFUL>
 __device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Raw raw, bool lane_ok, uint32_t xu,
                                            int P[4]) {
     const int o0 = f.bl_i[YODD * 2], o1 = f.bl_i[YODD * 2 + 1];
-    const bool row_ok = y >= 0 && y < f.h;
+    const bool row_ok = !VRIM || (y >= 0 && y < f.h);
     uint32_t v[4];
-    if (ODD && CAREFUL && y == f.h - 1) {
+    if (ODD && VRIM && y == f.h - 1) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = (lane_ok && x + k < f.w) ? raw_pixel<FMT>(f, xu + k, (uint32_t)y) : 0u;
     } else if (FMT == FMT_P12 || FMT == FMT_G10) {
@@ -567,14 +598,16 @@ __device__ __forceinline__ uint32_t lut_last(const FastParams& f) {
 
 // One step: output rows yA, yA + 1 (yA even) of the lane run at x.  Pw holds
 // the decoded rows yA-1 .. yA+4; Dw the MG colour differences of rows yA-1,
-// yA on entry, and the step forms rows yA+1, yA+2 into Dw[2], Dw[3].
-template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD, bool EPI>
+// yA on entry, and the step forms rows yA+1, yA+2 into Dw[2], Dw[3] (DGIVEN:
+// all four are given, and of Pw only the rows the step reads -- yA, yA+1 for
+// MG, yA-1 .. yA+2 for HQLI -- need to hold values).
+template <int FMT, int PAT, int MODE, bool CAREFUL, bool ODD, bool EPI, bool DGIVEN = false, int OUT = 0>
 __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, int x, int yA, uint32_t* stage,
                                           const int (*Pw)[4], int (*Dw)[3], const bool col_ok[4],
                                           const bool col_core[4], const uint16_t* lut) {
     const int w = f.w, h = f.h;
     const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
-    if (MODE != MODE_HQLI) {
+    if (MODE != MODE_HQLI && !DGIVEN) {
         d_row<PAT, MODE, CAREFUL>(&Pw[0], 1, (yA + 1 >= 2) && (yA + 1 < h - 2), col_ok, Dw[2]);
         d_row<PAT, MODE, CAREFUL>(&Pw[1], 0, (yA + 2 >= 2) && (yA + 2 < h - 2), col_ok, Dw[3]);
     }
@@ -647,6 +680,15 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
         const int y = yA + a;
         const int lane = threadIdx.x & 63;
         if (!CAREFUL) {
+            // u16 RGB in aligned strips (OUT 1: the single-frame kernel on a
+            // deeper-than-8-bit frame): every interior window is whole chunks
+            if (OUT == 1) {
+                int lo, hi;
+                seg_cols(f, sp, y, lo, hi);
+                store_row_u16_whole(f, stage, lane, (uint32_t)y * (uint32_t)w + (uint32_t)sp.xs, lo - sp.xs,
+                                    hi - sp.xs, R[a], G[a], B[a]);
+                continue;
+            }
             // staged rows need 4-byte aligned runs: always for u16, for u8 when w % 4 == 0
             if (!ODD || f.out16) {
                 int lo, hi;
@@ -694,7 +736,7 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
 
 // One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
 // Every load of the band is issued up front.
-template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false>
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false, bool VRIM = CAREFUL>
 __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage,
                                          const uint16_t* lut = nullptr) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
@@ -706,12 +748,12 @@ __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, i
     const uint32_t rs = row_stride<FMT>((uint32_t)f.w), lo = lane_offset<FMT>(xu);
     Raw raw[NR];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) raw[i] = load_row<FMT, CAREFUL, ODD>(f, y0 - 3 + i, xu, lane_ok, rs, lo);
+    for (int i = 0; i < NR; ++i) raw[i] = load_row<FMT, CAREFUL, ODD, VRIM>(f, y0 - 3 + i, xu, lane_ok, rs, lo);
 
     int P[NR][4];  // row y0 - 3 + i; y0 even, so row i is odd when i is even
     auto decode = [&](int i) {
-        if (i & 1) decode_row<FMT, CAREFUL, ODD, 0>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
-        else decode_row<FMT, CAREFUL, ODD, 1>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+        if (i & 1) decode_row<FMT, CAREFUL, ODD, 0, VRIM>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+        else decode_row<FMT, CAREFUL, ODD, 1, VRIM>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
     };
 #pragma unroll
     for (int i = 0; i < 6; ++i) decode(i);
@@ -729,35 +771,211 @@ __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, i
     }
 }
 
-}  // namespace
 
-// Wave -> (strip, band).  Blocks are spread round-robin over the 8 XCDs; the
-// remap gives each XCD a contiguous run of bands so that the 6-row aprons
-// shared by vertically adjacent bands are read through one L2.  False for the
-// waves past the last band.
-__device__ __forceinline__ bool wave_place(const FastParams& f, int& sx, int& by) {
-    const int wpb = blockDim.x >> 6;
-    const int nblk = gridDim.x;
-    // logical block: consecutive within an XCD (or the launch order).  With
-    // nblk = 8q + r, XCD x runs q + (x < r) blocks; the remap is a bijection
-    // of [0, nblk) for every nblk.
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int q8 = nblk >> 3, r8 = nblk & 7;
-    const int lblk = (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x;
-    // wave-uniform: keeps the band / strip and every row's address arithmetic in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(lblk * wpb + (int)(threadIdx.x >> 6));
-    if (lblk >= nblk || wave >= f.nwaves) return false;
-    // wave order: band-fastest (consecutive waves walk down a strip) or
-    // strip-fastest (consecutive waves cover a row band left to right)
-    const bool strip_fastest = (f.xcd_remap & 2) != 0;
-    by = strip_fastest ? wave / f.nsx : wave % f.nby;
-    sx = strip_fastest ? wave % f.nsx : wave / f.nby;
-    return true;
+// ------------------------------------------------------ single-frame groups
+//
+// The single-frame kernel runs workgroups of GB waves stacked down one strip:
+// wave W of group g computes rows yg + TH W .. yg + TH W + TH - 1 (yg =
+// g TH GB).  The group decodes each of its TH GB + 6 raw rows once and forms
+// each of its TH GB + 2 MG colour-difference rows once, sharing both through
+// LDS, where a wave on its own decodes TH + 6 rows and forms TH + 2 for TH
+// output rows:
+//   phase 1  wave W decodes rows W, W + GB, ... of yg - 3 .. yg + TH GB + 2
+//            into LDS as 16-bit sample pairs (P0, P2) and (P1, P3);
+//   phase 2  (MG) wave W forms colour-difference rows W, W + GB, ... of
+//            yg - 1 .. yg + TH GB from five LDS rows each, its two R/B sites
+//            per lane into LDS;
+//   phase 3  every wave reads its own rows and the colour differences around
+//            them and runs band_step (interpolation, border fill, epilogue,
+//            staged stores) as the per-wave form does.
+// Groups whose rows or columns touch the frame rim run the same phases with
+// the careful forms (masked loads, sentinels, border fill).
+
+__device__ __forceinline__ void lds_sync() { __syncthreads(); }
+
+// LDS layout of a group (bytes): decoded rows (512 B: x[64] then y[64]),
+// colour-difference rows (512 B: own site 0 [64] then own site 1 [64]), and,
+// for 2-row waves, separate 1536-byte staging slices.  With 4-row waves a
+// wave's staging slice aliases LDS no other wave reads in phase 3: its own
+// decoded rows under MG (read into registers first), the colour-difference
+// rows under HQLI (which has none).
+template <int TH, int GB>
+struct GroupLayout {
+    static constexpr int kRows = TH * GB;
+    static constexpr int kPR = kRows + 6;
+    static constexpr int kDR = kRows + 2;
+    static constexpr int kDOff = kPR * 512;
+    static constexpr int kSOff = kDOff + kDR * 512;
+    static constexpr bool kSepStage = TH < 4;
+    static constexpr int kBytes = kSOff + (kSepStage ? GB * 1536 : 0);
+};
+
+__device__ __forceinline__ void unpack4(uint32_t X, uint32_t Y, int p[4]) {
+    p[0] = (int)(X & 0xFFFFu);
+    p[2] = (int)(X >> 16);
+    p[1] = (int)(Y & 0xFFFFu);
+    p[3] = (int)(Y >> 16);
 }
 
-template <int FMT, int PAT, int MODE, int TH, bool ODD, bool EPI = false>
-__global__ __launch_bounds__(256) void k_fast(FastParams f) {
-    __shared__ uint4 s_stage[4][96];  // one 1536-byte row slice per wave
+// D[3] of a row (as d_row forms it) from the lane's own two sites
+template <int PAT>
+__device__ __forceinline__ void d_from_own(int py, int own0, int own1, int D[3]) {
+    if (rb_col_parity<PAT>(py) == 0) {
+        D[0] = own0; D[1] = own1; D[2] = from_right(own0);
+    } else {
+        D[0] = from_left(own1); D[1] = own0; D[2] = own1;
+    }
+}
+
+// One wave W (compile time) of an interior group.
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, int W>
+__device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp, int yg, uint8_t* L,
+                                           const uint16_t* lut) {
+    using G = GroupLayout<TH, GB>;
+    const int lane = threadIdx.x & 63;
+    const int x = sp.xs - 4 + 4 * lane;
+    const bool lane_ok = x >= 0 && x < f.w;
+    const uint32_t xu = lane_ok ? (uint32_t)x : 0u;
+    bool col_ok[4], col_core[4];
+    lane_columns(x, f.w, col_ok, col_core);
+    const uint32_t rs = row_stride<FMT>((uint32_t)f.w), lo = lane_offset<FMT>(xu);
+    uint32_t* Lw = (uint32_t*)L;  // dword view
+    // phase 1: decoded rows i = W + GB k (row yg - 3 + i, parity (W + 1) & 1)
+    constexpr int NI = (G::kPR - W + GB - 1) / GB;
+    constexpr int YODD = (W + 1) & 1;
+    {
+        Raw raw[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) raw[k] = load_row<FMT, false, ODD>(f, yg - 3 + W + GB * k, xu, lane_ok, rs, lo);
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = W + GB * k;
+            int P[4];
+            decode_row<FMT, false, ODD, YODD>(f, x, yg - 3 + i, raw[k], lane_ok, xu, P);
+            Lw[i * 128 + lane] = (uint32_t)P[0] | ((uint32_t)P[2] << 16);
+            Lw[i * 128 + 64 + lane] = (uint32_t)P[1] | ((uint32_t)P[3] << 16);
+        }
+    }
+    lds_sync();
+    // phase 2 (MG): colour-difference rows j = W + GB k (row yg - 1 + j,
+    // parity 1 - (W & 1)) from decoded rows j .. j + 4
+    if (MODE != MODE_HQLI) {
+        constexpr int NJ = (G::kDR - W + GB - 1) / GB;
+        constexpr int PY = 1 - (W & 1);
+        constexpr int PC = rb_col_parity<PAT>(PY);
+#pragma unroll
+        for (int k = 0; k < NJ; ++k) {
+            const int j = W + GB * k;
+            int P5[5][4];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                if (r == 2) {
+                    unpack4(Lw[(j + 2) * 128 + lane], Lw[(j + 2) * 128 + 64 + lane], P5[2]);
+                } else {  // only the R/B-site columns of this row's parity are read
+                    const uint32_t v = Lw[(j + r) * 128 + (PC == 0 ? 0 : 64) + lane];
+                    P5[r][PC] = (int)(v & 0xFFFFu);
+                    P5[r][PC + 2] = (int)(v >> 16);
+                    P5[r][1 - PC] = 0;
+                    P5[r][3 - PC] = 0;
+                }
+            }
+            int D[3];
+            d_row<PAT, MODE, false>(P5, PY, 1, col_ok, D);
+            Lw[G::kDOff / 4 + j * 128 + lane] = (uint32_t)(PC == 0 ? D[0] : D[1]);
+            Lw[G::kDOff / 4 + j * 128 + 64 + lane] = (uint32_t)(PC == 0 ? D[1] : D[2]);
+        }
+    }
+    lds_sync();
+    // phase 3: output rows yg + TH W .. yg + TH W + TH - 1 (inside the frame:
+    // interior groups end 3 rows above its last row)
+    const int yw = yg + TH * W;
+    constexpr int pw = TH * W + 3;  // decoded row of yw
+    uint32_t cx[TH], cy[TH];
+    if (MODE != MODE_HQLI) {
+#pragma unroll
+        for (int r = 0; r < TH; ++r) {
+            cx[r] = Lw[(pw + r) * 128 + lane];
+            cy[r] = Lw[(pw + r) * 128 + 64 + lane];
+        }
+    }
+    uint32_t* stage;
+    if (G::kSepStage) stage = Lw + G::kSOff / 4 + W * 384;
+    else if (MODE != MODE_HQLI) stage = Lw + pw * 128;  // own decoded rows, read above
+    else stage = Lw + G::kDOff / 4 + W * 384;            // HQLI: no colour-difference rows
+    // MG into u16 RGB without the epilogue: interpolation and whole-chunk
+    // staged stores inline; every other form through band_step
+    constexpr bool lean = MODE != MODE_HQLI && FMT != FMT_U8 && !EPI;
+    const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
+#pragma unroll
+    for (int s = 0; s < TH / 2; ++s) {
+        const int yA = yw + 2 * s;
+        int Pw[6][4];
+        if (MODE == MODE_HQLI) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)  // rows yA - 1 .. yA + 2
+                unpack4(Lw[(pw + 2 * s - 1 + r) * 128 + lane], Lw[(pw + 2 * s - 1 + r) * 128 + 64 + lane], Pw[r]);
+        } else {
+            unpack4(cx[2 * s], cy[2 * s], Pw[1]);
+            unpack4(cx[2 * s + 1], cy[2 * s + 1], Pw[2]);
+        }
+        int Dw[4][3];
+        if (MODE != MODE_HQLI) {
+            const int jb = TH * W + 2 * s;  // colour-difference row of yA - 1
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                d_from_own<PAT>((t + 1) & 1, (int)Lw[G::kDOff / 4 + (jb + t) * 128 + lane],
+                                (int)Lw[G::kDOff / 4 + (jb + t) * 128 + 64 + lane], Dw[t]);
+        }
+        if (lean) {
+            int R[2][4], Gc[2][4], B[2][4];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    mg_px<PAT, MODE>(a, k, Pw[1 + a][k], Dw[a], Dw[a + 1], Dw[a + 2], ch, R[a][k], Gc[a][k], B[a][k]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int y = yA + a;
+                int l0, h0;
+                seg_cols(f, sp, y, l0, h0);
+                store_row_u16_whole(f, stage, lane, (uint32_t)y * (uint32_t)f.w + (uint32_t)sp.xs, l0 - sp.xs,
+                                    h0 - sp.xs, R[a], Gc[a], B[a]);
+            }
+        } else {
+            band_step<FMT, PAT, MODE, false, ODD, EPI, true, FMT == FMT_U8 ? 2 : 1>(f, sp, x, yA, stage, Pw, Dw,
+                                                                                 col_ok, col_core, lut);
+        }
+    }
+}
+
+}  // namespace
+
+// Group -> (strip, group row).  Blocks are spread round-robin over the 8
+// XCDs; the remap (xcd_remap bit 0) gives each XCD a contiguous run of
+// groups, so the raw rows that vertically adjacent groups share are read
+// through one L2.  Strip-fastest (bit 1): consecutive groups cover a group
+// row left to right; else they walk down a strip.  The quotient uses the
+// host's reciprocal place_mul = ceil(2^40 / divisor), exact while block count
+// x divisor < 2^40 (the host leaves place_mul 0 otherwise: plain division).
+__device__ __forceinline__ void group_place(const FastParams& f, int& sx, int& g) {
+    const int nblk = gridDim.x;
+    // with nblk = 8q + r, XCD x runs q + (x < r) blocks: a bijection of [0, nblk)
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int q8 = nblk >> 3, r8 = nblk & 7;
+    const uint32_t lblk = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x);
+    const bool strip_fastest = (f.xcd_remap & 2) != 0;
+    const uint32_t d = strip_fastest ? (uint32_t)f.nsx : (uint32_t)f.nby;
+    const uint32_t q = f.place_mul ? (uint32_t)(((uint64_t)lblk * f.place_mul) >> 40) : lblk / d;
+    const uint32_t r = lblk - q * d;
+    g = (int)(strip_fastest ? q : r);
+    sx = (int)(strip_fastest ? r : q);
+}
+
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI = false>
+__global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
+    using G = GroupLayout<TH, GB>;
+    __shared__ uint4 s_group[G::kBytes / 16];
     // epilogue form: the part of the gamma table the demosaic's values can
     // reach (lut_last + 1 <= 4096 entries, checked on the host) in LDS,
     // filled by the whole workgroup before any wave leaves
@@ -766,32 +984,52 @@ __global__ __launch_bounds__(256) void k_fast(FastParams f) {
         for (int i = (int)threadIdx.x; i <= (int)lut_last(f); i += (int)blockDim.x) s_lut[i] = f.epi.lut[i];
         __syncthreads();
     }
-    int sx, by;
-    if (!wave_place(f, sx, by)) return;
+    int sx, g;
+    group_place(f, sx, g);
     const Strip sp = strip_of(f, sx);
-    const int y0 = by * TH;
-    // careful: the band's outputs touch the 3-pixel frame rim, or its raw
-    // window reaches the stream's final row
-    const bool careful = y0 < 3 || y0 + TH + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
-    uint32_t* stage = (uint32_t*)s_stage[threadIdx.x >> 6];
-    if (careful) run_band<FMT, PAT, MODE, true, TH, ODD, EPI>(f, sp, y0, stage, s_lut);
-    else run_band<FMT, PAT, MODE, false, TH, ODD, EPI>(f, sp, y0, stage, s_lut);
+    const int yg = g * G::kRows;
+    uint8_t* L = (uint8_t*)s_group;
+    // rim: the group's outputs touch the 3-pixel frame rim, or its raw
+    // window reaches the stream's final row.  Its waves take their bands one
+    // by one on the per-wave path (vrim: the band's rows reach the rim, else
+    // only the strip's columns do).
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const bool rim = yg < 3 || yg + G::kRows + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
+    if (rim) {
+        const int y0 = yg + TH * w;
+        if (y0 >= f.h) return;
+        uint32_t* stage = (uint32_t*)L + w * 384;  // 1536 bytes of the group's LDS per wave
+        const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
+        const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
+        if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true>(f, sp, y0, stage, s_lut);
+        else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false>(f, sp, y0, stage, s_lut);
+        else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false>(f, sp, y0, stage, s_lut);
+        return;
+    }
+    switch (w) {
+    case 0: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, 0>(f, sp, yg, L, s_lut); break;
+    case 1: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, 1>(f, sp, yg, L, s_lut); break;
+    case 2: if (GB > 2) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, (GB > 2 ? 2 : 0)>(f, sp, yg, L, s_lut); break;
+    default: if (GB > 3) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, (GB > 3 ? 3 : 0)>(f, sp, yg, L, s_lut); break;
+    }
 }
 
 // ---------------------------------------------------------------- launchers
 
 template <int FMT, int PAT, int MODE, bool ODD>
 static void launch_fmpo(const FastParams& f, hipStream_t st) {
-    const int blocks = (f.nwaves + f.wpb - 1) / f.wpb;
-    if (f.epi.flags) {  // the epilogue form is built for bands of 4 rows only
-        hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, ODD, true>), dim3(blocks), dim3(64 * f.wpb), 0, st, f);
+    // f.nsx strips x f.nby groups, f.wpb waves per group, f.th rows per wave
+    const int blocks = f.nsx * f.nby;
+    if (f.epi.flags) {  // the epilogue form is built for 4-row waves in groups of 4 only
+        hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD, true>), dim3(blocks), dim3(256), 0, st, f);
         return;
     }
-    switch (f.th) {
-    case 2: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    case 4: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    case 8: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 8, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
-    default: hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 16, ODD>), dim3(blocks), dim3(64 * f.wpb), 0, st, f); break;
+    if (f.th == 2) {
+        if (f.wpb == 2) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, 2, ODD>), dim3(blocks), dim3(128), 0, st, f);
+        else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
+    } else {
+        if (f.wpb == 2) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 2, ODD>), dim3(blocks), dim3(128), 0, st, f);
+        else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
     }
 }
 

@@ -10,21 +10,22 @@ namespace imop_hip {
 
 // Launch shape (set by the autotuner through imop_hip_set_launch_shape), one
 // word so that a launch never sees half of a concurrent change:
-// rows | wpb << 16 | xcd_remap << 24.  xcd_remap bit 0: XCD-contiguous block
-// order; bit 1: strip-fastest wave order.  rows 0 is the automatic band
-// height: 4 rows, except that a frame that would launch fewer than
-// kSmallFrameWaves waves with 4-row bands (under 4 per SIMD) uses 2-row
-// bands, twice the waves to cover the latency of a small frame.
+// rows | wpb << 16 | xcd_remap << 24.  rows: rows per wave (2 or 4; 0 =
+// automatic); wpb: waves per group (2 or 4).  xcd_remap bit 0:
+// XCD-contiguous group order; bit 1: strip-fastest group order.  Automatic
+// rows: 4, except that a frame that would launch fewer than kSmallFrameWaves
+// waves with 4-row waves (under 4 per SIMD) uses 2, twice the waves to cover
+// the latency of a small frame.
 static std::atomic<uint32_t> g_shape{0u | (4u << 16) | (3u << 24)};
 constexpr long kSmallFrameWaves = 4096;
 
 void set_launch_shape(int rows, int wpb, int xcd_remap) {
-    const bool rows_ok = rows == 0 || rows == 2 || rows == 4 || rows == 8 || rows == 16;
+    const bool rows_ok = rows == 0 || rows == 2 || rows == 4;
     uint32_t cur = g_shape.load();
     for (;;) {
         int r = (int)(cur & 0xFFFFu), w = (int)((cur >> 16) & 0xFFu), x = (int)(cur >> 24);
         if (rows_ok) r = rows;
-        if (wpb == 1 || wpb == 2 || wpb == 4) w = wpb;
+        if (wpb == 2 || wpb == 4) w = wpb;
         if (xcd_remap >= 0 && xcd_remap <= 3) x = xcd_remap;
         const uint32_t next = (uint32_t)r | ((uint32_t)w << 16) | ((uint32_t)x << 24);
         if (g_shape.compare_exchange_weak(cur, next)) return;
@@ -58,6 +59,7 @@ bool fast_path_ok(const FrameParams& f) {
     return true;
 }
 
+// Groups of wpb waves x th rows over nsx strips.
 FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     FastParams q{};
     q.in = f.in;
@@ -72,13 +74,18 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     q.out16 = f.out16;
     q.bl_on = f.bl_on;
     for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
-    q.th = (th == 2 || th == 4 || th == 8) ? th : 16;
-    q.wpb = wpb;
+    q.th = th == 2 ? 2 : 4;
+    q.wpb = wpb == 2 ? 2 : 4;
     q.xcd_remap = xcd;
     q.aligned = 1;
     q.nsx = strip_count(f.w, f.out16 != 0);
-    q.nby = (f.h + q.th - 1) / q.th;
-    q.nwaves = q.nsx * q.nby;
+    q.nby = (f.h + q.th * q.wpb - 1) / (q.th * q.wpb);  // group rows
+    q.nwaves = q.nsx * q.nby * q.wpb;
+    // group placement divides the block index by the strip or group-row count
+    // (exact while blocks x divisor < 2^40; else 0: the kernel divides)
+    const uint64_t d = (xcd & 2) ? (uint64_t)q.nsx : (uint64_t)q.nby;
+    const uint64_t blocks = (uint64_t)q.nsx * (uint64_t)q.nby;
+    q.place_mul = blocks * d < (1ull << 40) ? ((1ull << 40) + d - 1) / d : 0;
     return q;
 }
 
@@ -86,9 +93,13 @@ hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epi
     int rows, wpb, xcd;
     get_launch_shape(&rows, &wpb, &xcd);
     const bool with_epi = epi && epi->flags;
-    if (rows == 0)
-        rows = !with_epi && (long)strip_count(f.w, f.out16 != 0) * ((f.h + 3) / 4) < kSmallFrameWaves ? 2 : 4;
-    FastParams q = make_fast(f, with_epi ? 4 : rows, wpb, xcd);
+    if (with_epi) {  // the epilogue form: 4-row waves in groups of 4
+        rows = 4;
+        wpb = 4;
+    } else if (rows == 0) {
+        rows = (long)strip_count(f.w, f.out16 != 0) * ((f.h + 3) / 4) < kSmallFrameWaves ? 2 : 4;
+    }
+    FastParams q = make_fast(f, rows, wpb, xcd);
     if (with_epi) q.epi = *epi;
     switch (f.fmt) {
     case FMT_U8: return launch_fast_u8(q, mode, st);

@@ -1,9 +1,11 @@
 """Autotuner for the launch shape of the register-resident HQLI / MG kernel.
 
-The shape is (rows each wave walks down, waves per workgroup, XCD remap):
-rows sets the vertical apron overhead (6 extra raw rows per band) against
-the number of waves in flight; waves per workgroup and the XCD remap set how
-bands are spread over the 8 XCDs and their L2s.  Each candidate is timed on
+The shape is (rows each wave computes, waves per group, XCD remap): a group
+of waves stacked down one strip shares its decoded rows and colour
+differences through LDS, so rows x waves sets the vertical apron overhead
+(6 extra decoded rows per group) against the number of groups in flight;
+the XCD remap and group order set how groups are spread over the 8 XCDs and
+their L2s.  Each candidate is timed on
 device-resident synthetic frames of the requested class with HIP events,
 rotating enough buffer sets to defeat the 256 MB Infinity Cache, and the
 fastest is applied with imop_hip_set_launch_shape.
@@ -24,7 +26,7 @@ import numpy as np
 import imop_amd
 from imop_amd import scene
 
-CANDIDATES = list(itertools.product((2, 4, 8, 16), (1, 2, 4), (0, 1, 2, 3)))
+CANDIDATES = list(itertools.product((2, 4), (2, 4), (0, 1, 2, 3)))
 
 
 def frame_class(w: int, h: int, bd: int, packed: bool, algo: int) -> str:

@@ -107,12 +107,13 @@ def test_launch_shape_auto_rows_round_trip():
         assert imop_amd.get_launch_shape() == (0, 4, 3)
         imop_amd.set_launch_shape(*imop_amd.get_launch_shape())
         assert imop_amd.get_launch_shape() == (0, 4, 3)
-        imop_amd.set_launch_shape(8, 2, 1)
-        assert imop_amd.get_launch_shape() == (8, 2, 1)
+        imop_amd.set_launch_shape(2, 2, 1)
+        assert imop_amd.get_launch_shape() == (2, 2, 1)
         imop_amd.set_launch_shape(-1, 0, -1)
-        assert imop_amd.get_launch_shape() == (8, 2, 1)
-        imop_amd.set_launch_shape(3, 3, 7)
-        assert imop_amd.get_launch_shape() == (8, 2, 1)
+        assert imop_amd.get_launch_shape() == (2, 2, 1)
+        # rows 2 / 4 and groups of 2 / 4 waves only
+        imop_amd.set_launch_shape(8, 1, 7)
+        assert imop_amd.get_launch_shape() == (2, 2, 1)
         imop_amd.set_launch_shape(0)
         assert imop_amd.get_launch_shape() == (0, 2, 1)
     finally:

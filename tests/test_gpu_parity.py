@@ -197,7 +197,7 @@ def test_packed_stream_bound_ignores_larger_buffer(bd, algo, w, h):
     np.testing.assert_array_equal(rgb, ro)
 
 
-SHAPES = [(rows, wpb, xcd) for rows in (2, 4, 8, 16) for wpb in (1, 2, 4) for xcd in (0, 1, 2, 3)]
+SHAPES = [(rows, wpb, xcd) for rows in (2, 4) for wpb in (2, 4) for xcd in (0, 1, 2, 3)]
 
 
 @pytest.fixture
@@ -223,7 +223,7 @@ def test_every_launch_shape_is_bit_exact(shape, restore_launch_shape):
     e, rgb = gpu_run(raw, c["w"], c["h"], c["pattern"], c["algo"], c["bd"], c["packed"], bl, pc)
     assert e == 0
     assert GC.sha(rgb) == c["sha256"]
-    w, h = 1004, 118   # 5 strips; 59 / 30 / 15 / 8 bands
+    w, h = 1004, 118   # 5 strips; 15 / 8 / 8 / 4 groups (rows x waves 4 / 8 / 8 / 16)
     vals = scene.make_scene(h, w, 10, 0, seed=77)
     raw = scene.raw_bytes(vals, 10, True)
     for algo in (imop_amd.DemosaicAlgorithm.HQLI, imop_amd.DemosaicAlgorithm.MG):
