@@ -543,6 +543,128 @@ def bench_lens_shading(steps, warmup, with_cpu, sets=6):
     return out
 
 
+def bench_pipeline(steps, warmup, with_cpu, sets=6):
+    """The whole single-frame path on the headline frame: unpack -> black
+    level -> polynomial lens shading -> MG -> white balance / colour matrix /
+    gamma.  Fused (imop_hip_demosaic_pipeline: the lens gain applied in the
+    demosaic's decode, one launch) against the staged route the reference's
+    stage order suggests on a GPU (unpacked u16 frame in HBM -> in-place lens
+    pass -> demosaic + epilogue), and against demosaic + epilogue without the
+    lens stage.  Parity of the fused output against the oracle composition;
+    each form's `steps` calls captured into a HIP graph."""
+    import numpy as np
+    import torch
+
+    import imop_amd
+    from imop_amd import scene
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w, h, bd, packed, algo, bl, seed, lines = WORKLOADS["mg12"]
+    blv = [bl] * 4
+    vals = scene.make_scene(h, w, bd, 0, seed=seed, n_lines=lines)
+    raw = scene.raw_bytes(vals, bd, packed)
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(sets)]
+    d_out = [torch.empty(imop_amd.rgb_byte_size(w, h, bd), dtype=torch.uint8, device=dev) for _ in range(sets)]
+    # the staged route starts from unpacked, black-levelled u16 frames (the
+    # in-place lens pass rescales them on every call; the work per call is the same)
+    u16 = O.black_level(O.unpack(raw, w, h, bd, packed), 0, bd, blv)
+    d_u16 = [torch.from_numpy(u16.view(np.uint8).ravel().copy()).to(dev) for _ in range(sets)]
+    ep = imop_amd.rgb_epilogue(**ISP_EPILOGUE)
+    pro = imop_amd.bayer_prologue(LENS_SHADING["coef"], LENS_SHADING["center"])
+    lp = imop_amd.lens_shading_params(LENS_SHADING["coef"], LENS_SHADING["center"])
+
+    def fused(k):
+        if imop_amd.demosaic_pipeline(d_in[k % sets], d_out[k % sets], w, h, 0, algo, bd, packed, blv,
+                                      prologue=pro, epilogue=ep) != 0:
+            raise RuntimeError("imop_hip_demosaic_pipeline failed")
+
+    def staged(k):
+        if imop_amd.lens_shading(d_u16[k % sets], w, h, 0, bd, lp) != 0:
+            raise RuntimeError("imop_hip_lens_shading_poly failed")
+        if imop_amd.demosaic_isp(d_u16[k % sets], d_out[k % sets], w, h, 0, algo, bd, False, epilogue=ep) != 0:
+            raise RuntimeError("imop_hip_demosaic_isp failed")
+
+    def no_lens(k):
+        if imop_amd.demosaic_isp(d_in[k % sets], d_out[k % sets], w, h, 0, algo, bd, packed, blv, epilogue=ep) != 0:
+            raise RuntimeError("imop_hip_demosaic_isp failed")
+
+    fused(0)
+    torch.cuda.synchronize()
+    got = d_out[0].cpu().numpy().view(np.uint16).reshape(h, w, 3)
+    want = O.pipeline(raw, w, h, 0, int(algo), bd, packed, blv, False,
+                      (LENS_SHADING["coef"], LENS_SHADING["center"]),
+                      (7, ISP_EPILOGUE["gains"], ISP_EPILOGUE["ccm"], ISP_EPILOGUE["gamma"]))
+    parity = "bit-exact vs oracle" if np.array_equal(got, want) else "MISMATCH"
+
+    def graph_ms(fn):
+        for k in range(warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for k in range(steps):
+                fn(k)
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        while time.perf_counter() - t0 < 0.25:
+            g.replay()
+            torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    ms_fused = graph_ms(fused)
+    ms_staged = graph_ms(staged)
+    ms_nolens = graph_ms(no_lens)
+    log(f"bench: pipeline fused {ms_fused * 1e3:.1f} us, staged (u16 frame, lens pass, demosaic) "
+        f"{ms_staged * 1e3:.1f} us, without lens {ms_nolens * 1e3:.1f} us; parity {parity}")
+    alg = int(raw.size) + imop_amd.rgb_byte_size(w, h, bd)
+    out = {
+        "workload": "4024x3032 RGGB RAW12-packed: black level 64 -> polynomial lens shading -> MG demosaic -> "
+                    "white balance -> 3x3 colour matrix -> gamma 2.2 -> u16 RGB",
+        "value": round(w * h / (ms_fused / 1e3) / 1e6, 1),
+        "unit": "Mpix/s",
+        "ms_per_frame": round(ms_fused, 5),
+        "staged_u16_lens_pass_then_demosaic_ms": round(ms_staged, 5),
+        "without_lens_ms": round(ms_nolens, 5),
+        "speedup_vs_staged": round(ms_staged / ms_fused, 3),
+        "parity": parity,
+        "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
+                     "achieved": round(alg / (ms_fused / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (ms_fused / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "steps": steps,
+        "warmup": warmup,
+    }
+    if with_cpu:
+        ref = O.reference_available() and O.reference_isp_available()
+        fn = O.ref_pipeline if ref else O.pipeline
+        lens = (LENS_SHADING["coef"], LENS_SHADING["center"])
+        epi = (7, ISP_EPILOGUE["gains"], ISP_EPILOGUE["ccm"], ISP_EPILOGUE["gamma"])
+        threads = O.reference().ref_num_threads() if ref else O.oracle().orc_num_threads()
+        times = []
+        t_end = time.time() + 6.0
+        while len(times) < 2 or (time.time() < t_end and len(times) < 50):
+            t1 = time.perf_counter()
+            fn(raw, w, h, 0, int(algo), bd, packed, blv, False, lens, epi)
+            times.append(time.perf_counter() - t1)
+        times.sort()
+        med = times[len(times) // 2]
+        out["cpu_baseline"] = {"value": round(w * h / med / 1e6, 2), "unit": "Mpix/s", "cores": int(threads),
+                               "kind": "reference" if ref else "port",
+                               "sample": f"{len(times)} full {w}x{h} frames, median {med * 1e3:.0f} ms (unpack, "
+                                         f"black level, lens and the RGB passes on one thread each, the demosaic "
+                                         f"on {threads} threads, as the reference runs them)"}
+    return out
+
+
 def _defect_frame(kind, w, h, bd, seed):
     """A 12-bit test frame: a smooth scene with 0.1 % hot and 0.05 % cold
     pixels ("sparse"), or pure noise ("noise": corrections chained across the
@@ -1065,7 +1187,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch measurement")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-chain", action="store_true", help="skip the differentiable-chain measurement")
-    ap.add_argument("--only", choices=("batch", "stream", "chain", "isp", "lens", "defect", "analyze", "rcd", "sf", "l7", "ahd", "amaze",
+    ap.add_argument("--only", choices=("batch", "stream", "chain", "isp", "lens", "pipeline", "defect", "analyze", "rcd", "sf", "l7", "ahd", "amaze",
                                       "dfpd", "gige"),
                     default=None,
                     help="run only the batch, chain, ISP-epilogue, lens-shading or defect-correction measurement "
@@ -1110,6 +1232,11 @@ def main():
         return
     if args.only == "lens":
         out = bench_lens_shading(args.steps, args.warmup, False)
+        calibration_copies(args.copy_mb)
+        print(json.dumps(out), flush=True)
+        return
+    if args.only == "pipeline":
+        out = bench_pipeline(args.steps, args.warmup, False)
         calibration_copies(args.copy_mb)
         print(json.dumps(out), flush=True)
         return
@@ -1275,6 +1402,8 @@ def main():
         result["isp_epilogue"] = bench_isp(100, 5, not args.no_cpu_baseline)
         torch.cuda.empty_cache()
         result["lens_shading"] = bench_lens_shading(100, 5, not args.no_cpu_baseline)
+        torch.cuda.empty_cache()
+        result["pipeline"] = bench_pipeline(100, 5, not args.no_cpu_baseline)
         torch.cuda.empty_cache()
         result["defect_correct"] = bench_defect(30, 3, not args.no_cpu_baseline)
         torch.cuda.empty_cache()

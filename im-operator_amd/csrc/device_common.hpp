@@ -73,6 +73,22 @@ struct Epilogue {
     const uint16_t* lut;   // gamma table for every storable value 0 .. lut_max (device)
 };
 
+// Polynomial lens shading applied to the decoded, black-levelled samples
+// (lens_shading::process_polynomial, src/lens_shading/polynomial.cpp:11-52):
+// the reference's per-call constants cx, cy, max_r as the host computes them,
+// and the coefficient set of each CFA parity (y & 1) * 2 + (x & 1).
+struct LensParams {
+    float pc[4][3];     // {a2, a4, a6} of the channel at parity (y & 1) * 2 + (x & 1)
+    float cx, cy, max_r;
+    float inv_max_r;    // 1 / max_r correctly rounded
+    int32_t on;
+    // 1: the register-resident kernel's sequences are exact for every sample
+    // of the frame (lens_px in fast_impl.hpp): correctly rounded sqrt and
+    // division without range scaling, and products v * gain + 0.5 bounded
+    // inside (-2^31, 2^31) so the clamp is a median
+    int32_t fused_ok;
+};
+
 // Parameters of the register-resident HQLI / MG kernel (fast_kernels.hip).
 struct FastParams {
     const uint8_t* in;
@@ -90,6 +106,7 @@ struct FastParams {
     int32_t xcd_remap;    // give each XCD a contiguous run of waves
     int32_t aligned;      // aligned strips for u16 output (strip_stride)
     Epilogue epi;         // RGB stages applied before the stores (k_fast<.., EPI = true>)
+    LensParams lens;      // lens shading of the decoded samples (k_fast<.., EPI = true>)
 };
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }

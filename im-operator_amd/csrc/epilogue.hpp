@@ -83,6 +83,17 @@ hipError_t launch_rgb_epilogue(uint8_t* rgb, uint64_t npx, int out16, const Epil
 hipError_t launch_lens_shading(uint8_t* data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
                                float center_y, hipStream_t st);
 
+// The lens-shading constants of one frame (the reference's per-call cx, cy,
+// max_r; coefficient set per CFA parity); on = 1.
+LensParams make_lens_params(int w, int h, int pattern, const float coef[12], float center_x, float center_y);
+
+// Bayer prologue into an unpacked frame (u8 for bd <= 8, else u16): every
+// sample decoded from the input layout of f, black level when f.bl_on, then
+// the lens-shading gain -- unpack -> process_black_level ->
+// process_polynomial as the reference runs them, for the demosaic paths that
+// cannot take the gain in their decode.
+hipError_t launch_bayer_prologue(const FrameParams& f, const LensParams& lens, uint8_t* out, hipStream_t st);
+
 // Adaptive defect correction (defect_kernels.hip): the copy of the input
 // into `dst` (also clearing ctl[0..3]) and one fixed-point pass; tiles of
 // 128 x 32 pixels, `flags` holds 2 bytes per tile.

@@ -236,7 +236,7 @@ __device__ __forceinline__ void g10_samples(Raw r, int o0, int o1, uint32_t v[4]
 
 // Black level (integral offsets o0 / o1 for even / odd columns of the row)
 // or MSB alignment applied to 4 raw values.
-template <int FMT, bool MASK>
+template <int FMT, bool MASK, bool LENS = false>
 __device__ __forceinline__ void finish4(const FastParams& f, uint32_t v[4], bool ok, int o0, int o1, int p[4]) {
     if (FMT != FMT_U16) {
         // decoded samples never exceed max; offsets are 0 without black level
@@ -247,7 +247,9 @@ __device__ __forceinline__ void finish4(const FastParams& f, uint32_t v[4], bool
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             v[k] = (uint32_t)imin((int)__builtin_elementwise_sub_sat(v[k], (uint32_t)((k & 1) ? o1 : o0)), f.maxv);
-    } else if (f.bd < 16) {
+    } else if (f.bd < 16 && !LENS) {
+        // (with lens shading the sample goes to it as stored: the lens stage
+        // reads the unpacked buffer and clamps, polynomial.cpp:44-47)
         const int sh = 16 - f.bd;
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = (int)v[k] > f.maxv ? v[k] >> sh : v[k];
@@ -557,10 +559,68 @@ __device__ __forceinline__ Raw load_row(const FastParams& f, int y, uint32_t xu,
     return load_raw<FMT>(f, off, !CAREFUL || (row_ok && lane_ok));
 }
 
-// Decoded, black-levelled samples of row y (parity YODD) from its raw bytes.
-template <int FMT, bool CAREFUL, bool ODD, int YODD, bool VRIM = CAREFUL>
+// sqrt(t) rounded to nearest for t == 0 or 2^-96 <= t < inf: the hardware
+// estimate (within 1 ulp) replaced by its lower or upper neighbour when the
+// fused residual t - neighbour * estimate says that one is nearer (the
+// correction the compiler's correctly rounded sqrtf applies, without the
+// rescaling it needs only below 2^-96).
+__device__ __forceinline__ float sqrt_rn(float t) {
+    const float s = __builtin_amdgcn_sqrtf(t);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float lo = fmaf(-sm, s, t) <= 0.0f ? sm : s;
+    return fmaf(-sp, s, t) > 0.0f ? sp : lo;
+}
+
+// a / b rounded to nearest from y = 1/b rounded to nearest: two fused
+// remainder corrections of a * y (Markstein); exact while no intermediate
+// is subnormal or overflows.
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+#pragma clang fp contract(off)
+    const float q0 = a * y;
+    const float q1 = fmaf(fmaf(-b, q0, a), y, q0);
+    return fmaf(fmaf(-b, q1, a), y, q1);
+}
+
+// Lens shading of one decoded, black-levelled sample at (x, y) of CFA parity
+// `parity`: the reference's operations (polynomial.cpp:30-47) in its order
+// and contractions -- r = sqrt(fma(dx, dx, dy * dy)) / max_r, gain =
+// fma(a6, r^6, fma(a4, r^4, fma(a2, r^2, 1))), v' = clamp(int(fma(v, gain,
+// 0.5)), 0, max) -- with sqrt_rn / div_rn for the rounded sqrt and quotient
+// and the clamp as a median (the host takes this path only for frames where
+// those are exact: LensParams::fused_ok).
+__device__ __forceinline__ int lens_px(const FastParams& f, int v, int x, float dyy, int parity) {
+#pragma clang fp contract(off)
+    const float dx = (float)x - f.lens.cx;
+    const float r = div_rn(sqrt_rn(fmaf(dx, dx, dyy)), f.lens.max_r, f.lens.inv_max_r);
+    const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
+    const float* a = f.lens.pc[parity];
+    const float gain = fmaf(a[2], r6, fmaf(a[1], r4, fmaf(a[0], r2, 1.0f)));
+    return (int)__builtin_amdgcn_fmed3f(fmaf((float)v, gain, 0.5f), 0.0f, (float)f.maxv);
+}
+
+template <int FMT, bool CAREFUL, bool ODD, int YODD, bool VRIM, bool LENS>
+__device__ __forceinline__ void decode_row_core(const FastParams& f, int x, int y, Raw raw, bool lane_ok,
+                                                uint32_t xu, int P[4]);
+
+// Decoded, black-levelled samples of row y (parity YODD) from its raw bytes;
+// LENS: then the lens-shading gain (the fused Bayer prologue).
+template <int FMT, bool CAREFUL, bool ODD, int YODD, bool VRIM = CAREFUL, bool LENS = false>
 __device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Raw raw, bool lane_ok, uint32_t xu,
                                            int P[4]) {
+    decode_row_core<FMT, CAREFUL, ODD, YODD, VRIM, LENS>(f, x, y, raw, lane_ok, xu, P);
+    if (LENS) {
+#pragma clang fp contract(off)
+        const float dy = (float)y - f.lens.cy;
+        const float dyy = dy * dy;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[k] = lens_px(f, P[k], x + k, dyy, YODD * 2 + (k & 1));
+    }
+}
+
+template <int FMT, bool CAREFUL, bool ODD, int YODD, bool VRIM, bool LENS>
+__device__ __forceinline__ void decode_row_core(const FastParams& f, int x, int y, Raw raw, bool lane_ok,
+                                                uint32_t xu, int P[4]) {
     const int o0 = f.bl_i[YODD * 2], o1 = f.bl_i[YODD * 2 + 1];
     const bool row_ok = !VRIM || (y >= 0 && y < f.h);
     uint32_t v[4];
@@ -577,7 +637,7 @@ __device__ __forceinline__ void decode_row(const FastParams& f, int x, int y, Ra
     } else {
         raw_values<FMT>(raw, ODD && YODD, v);  // w % 4 == 2: odd rows start at sub-pixel 2
     }
-    finish4<FMT, CAREFUL>(f, v, row_ok && lane_ok, o0, o1, P);
+    finish4<FMT, CAREFUL, LENS>(f, v, row_ok && lane_ok, o0, o1, P);
 }
 
 // Interior site columns of the lane run (MG green kernel) and the columns the
@@ -736,7 +796,8 @@ __device__ __forceinline__ void band_step(const FastParams& f, const Strip& sp, 
 
 // One band of TH rows from y0 (even) for the lane run at x.  ODD: w % 4 == 2.
 // Every load of the band is issued up front.
-template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false, bool VRIM = CAREFUL>
+template <int FMT, int PAT, int MODE, bool CAREFUL, int TH, bool ODD, bool EPI = false, bool VRIM = CAREFUL,
+          bool LENS = false>
 __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, int y0, uint32_t* stage,
                                          const uint16_t* lut = nullptr) {
     constexpr int NR = TH + 6;  // raw rows y0-3 .. y0+TH+2
@@ -752,8 +813,8 @@ __device__ __forceinline__ void run_band(const FastParams& f, const Strip& sp, i
 
     int P[NR][4];  // row y0 - 3 + i; y0 even, so row i is odd when i is even
     auto decode = [&](int i) {
-        if (i & 1) decode_row<FMT, CAREFUL, ODD, 0, VRIM>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
-        else decode_row<FMT, CAREFUL, ODD, 1, VRIM>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+        if (i & 1) decode_row<FMT, CAREFUL, ODD, 0, VRIM, LENS>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
+        else decode_row<FMT, CAREFUL, ODD, 1, VRIM, LENS>(f, x, y0 - 3 + i, raw[i], lane_ok, xu, P[i]);
     };
 #pragma unroll
     for (int i = 0; i < 6; ++i) decode(i);
@@ -828,7 +889,7 @@ __device__ __forceinline__ void d_from_own(int py, int own0, int own1, int D[3])
 }
 
 // One wave W (compile time) of an interior group.
-template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, int W>
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, bool LENS, int W>
 __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp, int yg, uint8_t* L,
                                            const uint16_t* lut) {
     using G = GroupLayout<TH, GB>;
@@ -851,7 +912,7 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
         for (int k = 0; k < NI; ++k) {
             const int i = W + GB * k;
             int P[4];
-            decode_row<FMT, false, ODD, YODD>(f, x, yg - 3 + i, raw[k], lane_ok, xu, P);
+            decode_row<FMT, false, ODD, YODD, false, LENS>(f, x, yg - 3 + i, raw[k], lane_ok, xu, P);
             Lw[i * 128 + lane] = (uint32_t)P[0] | ((uint32_t)P[2] << 16);
             Lw[i * 128 + 64 + lane] = (uint32_t)P[1] | ((uint32_t)P[3] << 16);
         }
@@ -972,7 +1033,7 @@ __device__ __forceinline__ void group_place(const FastParams& f, int& sx, int& g
     sx = (int)(strip_fastest ? r : q);
 }
 
-template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI = false>
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI = false, bool LENS = false>
 __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
     using G = GroupLayout<TH, GB>;
     __shared__ uint4 s_group[G::kBytes / 16];
@@ -1001,16 +1062,16 @@ __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
         uint32_t* stage = (uint32_t*)L + w * 384;  // 1536 bytes of the group's LDS per wave
         const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
         const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
-        if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true>(f, sp, y0, stage, s_lut);
-        else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false>(f, sp, y0, stage, s_lut);
-        else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false>(f, sp, y0, stage, s_lut);
+        if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true, LENS>(f, sp, y0, stage, s_lut);
+        else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
+        else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
         return;
     }
     switch (w) {
-    case 0: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, 0>(f, sp, yg, L, s_lut); break;
-    case 1: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, 1>(f, sp, yg, L, s_lut); break;
-    case 2: if (GB > 2) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, (GB > 2 ? 2 : 0)>(f, sp, yg, L, s_lut); break;
-    default: if (GB > 3) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, (GB > 3 ? 3 : 0)>(f, sp, yg, L, s_lut); break;
+    case 0: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 0>(f, sp, yg, L, s_lut); break;
+    case 1: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 1>(f, sp, yg, L, s_lut); break;
+    case 2: if (GB > 2) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 2 ? 2 : 0)>(f, sp, yg, L, s_lut); break;
+    default: if (GB > 3) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 3 ? 3 : 0)>(f, sp, yg, L, s_lut); break;
     }
 }
 
@@ -1020,7 +1081,12 @@ template <int FMT, int PAT, int MODE, bool ODD>
 static void launch_fmpo(const FastParams& f, hipStream_t st) {
     // f.nsx strips x f.nby groups, f.wpb waves per group, f.th rows per wave
     const int blocks = f.nsx * f.nby;
-    if (f.epi.flags) {  // the epilogue form is built for 4-row waves in groups of 4 only
+    // the ISP forms (RGB epilogue, with or without the lens prologue): 4-row waves in groups of 4
+    if (f.lens.on) {
+        hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD, true, true>), dim3(blocks), dim3(256), 0, st, f);
+        return;
+    }
+    if (f.epi.flags) {
         hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD, true>), dim3(blocks), dim3(256), 0, st, f);
         return;
     }

@@ -89,10 +89,10 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     return q;
 }
 
-hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epilogue* epi) {
+hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epilogue* epi, const LensParams* lens) {
     int rows, wpb, xcd;
     get_launch_shape(&rows, &wpb, &xcd);
-    const bool with_epi = epi && epi->flags;
+    const bool with_epi = (epi && epi->flags) || (lens && lens->on);
     if (with_epi) {  // the epilogue form: 4-row waves in groups of 4
         rows = 4;
         wpb = 4;
@@ -100,7 +100,8 @@ hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epi
         rows = (long)strip_count(f.w, f.out16 != 0) * ((f.h + 3) / 4) < kSmallFrameWaves ? 2 : 4;
     }
     FastParams q = make_fast(f, rows, wpb, xcd);
-    if (with_epi) q.epi = *epi;
+    if (epi && epi->flags) q.epi = *epi;
+    if (lens && lens->on) q.lens = *lens;
     switch (f.fmt) {
     case FMT_U8: return launch_fast_u8(q, mode, st);
     case FMT_U16: return launch_fast_u16(q, mode, st);

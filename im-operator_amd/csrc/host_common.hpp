@@ -41,6 +41,10 @@ FrameParams make_params(const void* in, uint64_t in_bytes, void* out, int w, int
 int launch_frame(const FrameParams& f, int algo, int mode, void* prism_ws, hipStream_t st,
                  const Epilogue* epi = nullptr);
 
+// A stream-ordered allocation of `bytes` on `st` (the library's pool when
+// eager, a graph node under capture); release it with hipFreeAsync on `st`.
+hipError_t planes_alloc(size_t bytes, hipStream_t st, void** out);
+
 // Checks, parameters and launch of one frame from device buffers
 // (imop_hip_demosaic; with an epilogue, imop_hip_demosaic_isp).
 int run_demosaic(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int w, int h, int pattern,

@@ -206,6 +206,52 @@ int imop_hip_demosaic_isp(const void* d_in, uint64_t in_bytes, void* d_out, uint
                         black_level, black_level_mode, (hipStream_t)stream, ep.flags ? &ep : nullptr);
 }
 
+int imop_hip_demosaic_pipeline(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
+                               int height, int pattern, int algorithm, int bit_depth, int is_packed,
+                               const float* black_level, int black_level_mode,
+                               const imop_hip_bayer_prologue* prologue, const imop_hip_rgb_epilogue* epilogue,
+                               void* stream) {
+    if (device_count_cached() == 0) return kInternal;
+    int e = check_args(d_in, d_out, width, height, pattern, algorithm, bit_depth, is_packed, black_level,
+                       black_level_mode);
+    if (e != kOk) return e;
+    Epilogue ep;
+    e = make_epilogue(epilogue, bit_depth, &ep);
+    if (e != kOk) return e;
+    hipStream_t st = (hipStream_t)stream;
+    const Epilogue* epp = ep.flags ? &ep : nullptr;
+    if (!prologue || prologue->flags == 0)
+        return run_demosaic(d_in, in_bytes, d_out, out_bytes, width, height, pattern, algorithm, bit_depth, is_packed,
+                            black_level, black_level_mode, st, epp);
+    if (prologue->flags & ~1) return kInternal;
+    // run_demosaic's buffer checks
+    if (in_bytes < bayer_bytes(width, height, bit_depth, is_packed) || out_bytes < rgb_bytes(width, height, bit_depth))
+        return kInternal;
+    if (bit_depth > 8 && ((!is_packed && ((uintptr_t)d_in & 1)) || ((uintptr_t)d_out & 1))) return kInternal;
+    const LensParams lens = make_lens_params(width, height, pattern, &prologue->lens.coef[0][0],
+                                             prologue->lens.center_x, prologue->lens.center_y);
+    int mode = -1;
+    const FrameParams f = make_params(d_in, in_bytes, d_out, width, height, pattern, bit_depth, is_packed,
+                                      black_level, black_level_mode, algorithm, &mode);
+    // the gain in the register-resident kernel's decode, for centres and
+    // coefficients where its sequences are exact (its gamma table in LDS: at
+    // most 4096 entries)
+    const uint32_t lut_entries = f.out16 ? (uint32_t)f.maxv + 1u : 256u;
+    if ((algorithm == kHqli || algorithm == kMg) && lens.fused_ok && fast_path_ok(f) &&
+        (!(ep.flags & EPI_GAMMA) || lut_entries <= 4096u))
+        return launch_fast(f, mode, st, epp, &lens) == hipSuccess ? kOk : kInternal;
+    // else the prologue into a stream-ordered unpacked frame, demosaiced from there
+    const uint64_t ub = bayer_bytes(width, height, bit_depth, 0);
+    void* tmp = nullptr;
+    if (planes_alloc(ub, st, &tmp) != hipSuccess) return kInternal;
+    e = launch_bayer_prologue(f, lens, (uint8_t*)tmp, st) == hipSuccess ? kOk : kInternal;
+    if (e == kOk)
+        e = run_demosaic(tmp, ub, d_out, out_bytes, width, height, pattern, algorithm, bit_depth, 0, nullptr, 0, st,
+                         epp);
+    const hipError_t fe = hipFreeAsync(tmp, st);
+    return e != kOk ? e : (fe == hipSuccess ? kOk : kInternal);
+}
+
 int imop_hip_rgb_epilogue_apply(void* d_rgb, uint64_t rgb_bytes_, int width, int height, int bit_depth,
                                 const imop_hip_rgb_epilogue* epilogue, void* stream) {
     if (device_count_cached() == 0) return kInternal;

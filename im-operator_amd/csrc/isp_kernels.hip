@@ -4,6 +4,8 @@
 //    registers (PRISM, the generic LDS-tiled kernels) run it after the
 //    demosaic; it is also the standalone imop_hip_rgb_epilogue_apply;
 //  * the polynomial lens-shading correction over an unpacked Bayer frame.
+#include <cmath>
+
 #include "epilogue.hpp"
 
 namespace imop_hip {
@@ -124,7 +126,99 @@ __global__ __launch_bounds__(256) void k_lens_shading(uint8_t* __restrict__ data
     }
 }
 
+// Bayer prologue: decode (no MSB alignment: the lens stage reads the
+// stored value), black level, lens gain; one sample per thread.
+__global__ __launch_bounds__(256) void k_bayer_prologue(FrameParams f, LensParams lens, uint8_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)blockIdx.y;
+    if (x >= f.w) return;
+    for (int yy = y; yy < f.h; yy += (int)gridDim.y) {
+        const uint64_t i = (uint64_t)yy * (uint64_t)f.w + (uint64_t)x;
+        int v;
+        switch (f.fmt) {
+        case FMT_U8: v = f.in[i]; break;
+        case FMT_U16: v = (int)((const uint16_t*)f.in)[i]; break;
+        case FMT_P12: v = decode_p12(f.in, i, f.in_bytes); break;
+        case FMT_P10: v = decode_p10(f.in, i, f.in_bytes); break;
+        case FMT_G10: v = decode_g10(f.in, i, f.in_bytes); break;
+        default: v = decode_pgen(f.in, i, f.in_bytes, f.bd); break;
+        }
+        const int par = ((yy & 1) << 1) | (x & 1);
+        if (f.bl_on) v = iclamp((int)((float)v - f.bl_q[par]), 0, f.maxv);
+        const float dy = (float)yy - lens.cy;
+        const float dyy = dy * dy;
+        const float dx = (float)x - lens.cx;
+        const float r = sqrtf(fmaf(dx, dx, dyy)) / lens.max_r;
+        const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2;
+        const float gain = fmaf(lens.pc[par][2], r6, fmaf(lens.pc[par][1], r4, fmaf(lens.pc[par][0], r2, 1.0f)));
+        const int o = epi_round_clamp<false>(fmaf((float)v, gain, 0.5f), f.maxv);
+        if (f.bd > 8) ((uint16_t*)out)[i] = (uint16_t)o;
+        else out[i] = (uint8_t)o;
+    }
+}
+
 }  // namespace
+
+LensParams make_lens_params(int w, int h, int pattern, const float coef[12], float center_x, float center_y) {
+#pragma clang fp contract(off)
+    LensParams p{};
+    // the reference's per-call constants (polynomial.cpp:26-28), GCC's contraction kept
+    p.cx = center_x * (float)w;
+    p.cy = center_y * (float)h;
+    p.max_r = sqrtf(fmaf(p.cx, p.cx, p.cy * p.cy));
+    p.on = 1;
+    // lens_shading::detail::bayer_color (common.hpp:28-33) per parity
+    static const int off[4][4] = {{0, 0, 1, 1}, {1, 1, 0, 0}, {0, 1, 1, 0}, {1, 0, 0, 1}};  // r_row r_col b_row b_col
+    const int* o = off[(pattern >= 0 && pattern < 4) ? pattern : 0];
+    for (int py = 0; py < 2; ++py)
+        for (int px = 0; px < 2; ++px) {
+            const int c = (py == o[0] && px == o[1]) ? 0 : (py == o[2] && px == o[3]) ? 3 : py == o[0] ? 1 : 2;
+            for (int k = 0; k < 3; ++k) p.pc[py * 2 + px][k] = coef[3 * c + k];
+        }
+    // 1 / max_r rounded to nearest: the double quotient's float, or the
+    // neighbour whose product with max_r is closer to 1 (products exact in double)
+    if (p.max_r > 0.0f && std::isfinite(p.max_r)) {
+        const float y = (float)(1.0 / (double)p.max_r);
+        float best = y;
+        double err = std::fabs((double)p.max_r * (double)y - 1.0);
+        for (const float c : {std::nextafter(y, 0.0f), std::nextafter(y, INFINITY)}) {
+            const double e = std::fabs((double)p.max_r * (double)c - 1.0);
+            if (e < err) { err = e; best = c; }
+        }
+        p.inv_max_r = best;
+    }
+    // Range of the register-resident kernel's sequences.  A centre
+    // coordinate of 0 or of magnitude in [2^-20, 2^40] keeps every nonzero
+    // sum of squares at or above 2^-86 (no scaling for the sqrt) and every
+    // quotient r normal; the gain bound keeps v * gain + 0.5 below 2^31.
+    auto centre_ok = [](float c) {
+        const float a = std::fabs(c);
+        return a == 0.0f || (a >= 0x1p-20f && a <= 0x1p40f);
+    };
+    bool ok = centre_ok(p.cx) && centre_ok(p.cy) && p.max_r > 0.0f && std::isfinite(p.max_r);
+    if (ok) {
+        const double dx = std::fmax(std::fabs((double)p.cx), std::fabs((double)(w - 1) - (double)p.cx));
+        const double dy = std::fmax(std::fabs((double)p.cy), std::fabs((double)(h - 1) - (double)p.cy));
+        const double R = std::sqrt(dx * dx + dy * dy) / (double)p.max_r * (1.0 + 1e-5);
+        const double R2 = R * R;
+        double g = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            const double gc = 1.0 + std::fabs((double)coef[3 * c]) * R2 + std::fabs((double)coef[3 * c + 1]) * R2 * R2 +
+                              std::fabs((double)coef[3 * c + 2]) * R2 * R2 * R2;
+            g = std::fmax(g, gc);
+        }
+        ok = std::isfinite(g) && 65535.0 * g * (1.0 + 1e-4) + 2.0 < 2147483648.0;
+    }
+    p.fused_ok = ok ? 1 : 0;
+    return p;
+}
+
+hipError_t launch_bayer_prologue(const FrameParams& f, const LensParams& lens, uint8_t* out, hipStream_t st) {
+    const dim3 grid((unsigned)((f.w + 255) / 256), (unsigned)(f.h < 65535 ? f.h : 65535)), block(256);
+    hipLaunchKernelGGL(k_bayer_prologue, grid, block, 0, st, f, lens, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_lens_shading(uint8_t* data, int w, int h, int pattern, int bd, const float coef[12], float center_x,
                                float center_y, hipStream_t st) {

@@ -13,7 +13,8 @@ hipError_t launch_plain(const FrameParams& f, int mode, hipStream_t st);
 // fast_path_ok(); launch shape from set_launch_shape().
 bool fast_path_ok(const FrameParams& f);
 // epi (optional): RGB stages applied in the same launch (bands of 4 rows).
-hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epilogue* epi = nullptr);
+hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epilogue* epi = nullptr,
+                       const LensParams* lens = nullptr);
 void set_launch_shape(int rows, int wpb, int xcd_remap);
 void get_launch_shape(int* rows, int* wpb, int* xcd_remap);
 

@@ -134,6 +134,24 @@ def lens_shading_params(coef, center=(0.5, 0.5)) -> LensShadingParams:
     return p
 
 
+class BayerPrologue(ctypes.Structure):
+    """imop_hip_bayer_prologue (include/imop_hip.h): flag 1 polynomial lens
+    shading with `lens`."""
+    _fields_ = [
+        ("flags", ctypes.c_int32),
+        ("lens", LensShadingParams),
+    ]
+
+
+def bayer_prologue(lens_coef=None, lens_center=(0.5, 0.5)) -> BayerPrologue:
+    """Prologue parameters; lens shading is on when its coefficients are given."""
+    p = BayerPrologue()
+    if lens_coef is not None:
+        p.flags |= 1
+        p.lens = lens_shading_params(lens_coef, lens_center)
+    return p
+
+
 class DataInfo(ctypes.Structure):
     """imop_hip_data_info (include/imop_hip.h): imop::DataInfo
     (include/imop/types.hpp:76-86) with up to 8 possible dimensions."""
@@ -198,6 +216,9 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_batch_plan_destroy.restype = i
         L.imop_hip_demosaic_isp.argtypes = [vp, u64, vp, u64, i, i, i, i, i, i, fp, i, ctypes.POINTER(RgbEpilogue), vp]
         L.imop_hip_demosaic_isp.restype = i
+        L.imop_hip_demosaic_pipeline.argtypes = [vp, u64, vp, u64, i, i, i, i, i, i, fp, i,
+                                                 ctypes.POINTER(BayerPrologue), ctypes.POINTER(RgbEpilogue), vp]
+        L.imop_hip_demosaic_pipeline.restype = i
         L.imop_hip_rgb_epilogue_apply.argtypes = [vp, u64, i, i, i, ctypes.POINTER(RgbEpilogue), vp]
         L.imop_hip_rgb_epilogue_apply.restype = i
         L.imop_hip_lens_shading_poly.argtypes = [vp, u64, i, i, i, i, ctypes.POINTER(LensShadingParams), vp]
@@ -306,6 +327,24 @@ def demosaic_isp(d_in, d_out, width: int, height: int, pattern: int, algorithm: 
                                        ctypes.c_void_p(d_out.data_ptr()), d_out.numel() * d_out.element_size(),
                                        width, height, int(pattern), int(algorithm), bit_depth,
                                        _packing(is_packed), blp, mode, ep, ctypes.c_void_p(_stream_arg(stream)))
+
+
+def demosaic_pipeline(d_in, d_out, width: int, height: int, pattern: int, algorithm: int, bit_depth: int = 8,
+                      is_packed: bool = False, black_level=None, per_channel: bool = False,
+                      prologue: BayerPrologue | None = None, epilogue: RgbEpilogue | None = None,
+                      stream=None) -> int:
+    """imop_hip_demosaic_pipeline: unpack, black level, the Bayer prologue
+    (lens shading), the demosaic and the RGB epilogue in one call on device
+    tensors."""
+    bl, mode = _bl_args(black_level, per_channel)
+    blp = bl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if bl is not None else None
+    pp = ctypes.byref(prologue) if prologue is not None else None
+    ep = ctypes.byref(epilogue) if epilogue is not None else None
+    return lib().imop_hip_demosaic_pipeline(ctypes.c_void_p(d_in.data_ptr()), d_in.numel() * d_in.element_size(),
+                                            ctypes.c_void_p(d_out.data_ptr()), d_out.numel() * d_out.element_size(),
+                                            width, height, int(pattern), int(algorithm), bit_depth,
+                                            _packing(is_packed), blp, mode, pp, ep,
+                                            ctypes.c_void_p(_stream_arg(stream)))
 
 
 def rgb_epilogue_apply(d_rgb, width: int, height: int, bit_depth: int, epilogue: RgbEpilogue, stream=None) -> int:

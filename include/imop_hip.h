@@ -305,6 +305,35 @@ typedef struct imop_hip_lens_shading {
 int imop_hip_lens_shading_poly(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
                                const imop_hip_lens_shading* params, void* stream);
 
+/* Bayer-domain stages between the black level and the demosaic.
+ *   flag 1: polynomial lens shading with `lens`,
+ *           lens_shading::process_polynomial (src/lens_shading/
+ *           polynomial.cpp:11-52).
+ * Other flag bits are reserved (InternalError). */
+typedef struct imop_hip_bayer_prologue {
+    int32_t flags;
+    struct imop_hip_lens_shading lens;
+} imop_hip_bayer_prologue;
+
+/* The whole single-frame path in one call: unpack -> black level -> the
+ * prologue -> demosaic -> the RGB epilogue, equal to the reference's CPU
+ * functions run in that order on the unpacked buffer (unpack,
+ * process_black_level, process_polynomial, demosaic_cpu, process_manual_wb,
+ * process_linear_3x3, process_gamma).  HQLI / MG frames on the
+ * register-resident path apply the lens gain to each sample as it is
+ * decoded, in the same launch as the demosaic and the epilogue (no unpacked
+ * frame is written); every other frame gets one extra pass that writes the
+ * prologue's result to a stream-ordered unpacked buffer, demosaiced from
+ * there.  prologue or epilogue NULL (or flags 0): that side is absent.
+ * Replaces lens_shading::process_lens_shading_cuda (src/gpu_dispatch.cpp:
+ * 455-486) + imop::demosaic_cuda + the RGB stages' GPU hooks, each a host
+ * round trip in the reference. */
+int imop_hip_demosaic_pipeline(const void* d_in, uint64_t in_bytes, void* d_out, uint64_t out_bytes, int width,
+                               int height, int pattern, int algorithm, int bit_depth, int is_packed,
+                               const float* black_level, int black_level_mode,
+                               const imop_hip_bayer_prologue* prologue, const imop_hip_rgb_epilogue* epilogue,
+                               void* stream);
+
 /* ---------------------------------------------- adaptive defect correction */
 
 /* Adaptive defect correction in place on an unpacked Bayer frame in device

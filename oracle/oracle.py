@@ -325,6 +325,79 @@ def unpack(raw: np.ndarray, w: int, h: int, bd: int, packed: bool) -> np.ndarray
     return (out.view(np.uint16) if bd > 8 else out).reshape(h, w)
 
 
+def black_level(bayer: np.ndarray, pattern: int, bd: int, offsets, per_channel: bool = False) -> np.ndarray:
+    """Oracle black level (process_global / process_per_channel) on an
+    unpacked (h, w) u8 / u16 frame; returns a copy."""
+    out = np.ascontiguousarray(bayer).copy()
+    h, w = out.shape
+    bl = np.asarray(offsets, dtype=np.float32).ravel()
+    if bl.size == 1:
+        bl = np.repeat(bl, 4)
+    bl = np.ascontiguousarray(bl, dtype=np.float32)
+    oracle().orc_black_level(out.view(np.uint8).ctypes.data_as(_u8p), w, h, pattern, bd, bl.ctypes.data_as(_fp),
+                             int(per_channel))
+    return out
+
+
+def pipeline(raw: np.ndarray, w: int, h: int, pattern: int, algo: int, bd: int, packed: bool = False,
+             black_level_=None, per_channel: bool = False, lens=None, epilogue=None) -> np.ndarray:
+    """The single-frame path as the reference's CPU functions in order:
+    unpack (packed input only) -> black level -> polynomial lens shading (lens = (coef, center))
+    -> demosaic -> white balance / colour matrix / gamma (epilogue = (flags,
+    gains, ccm, gamma)).  Returns the RGB array."""
+    if packed:
+        u = unpack(raw, w, h, bd, packed)
+    else:  # unpacked words go to the Bayer stages as stored (no MSB alignment)
+        u = np.ascontiguousarray(raw).view(np.uint8).ravel()[: w * h * (2 if bd > 8 else 1)].copy()
+        u = (u.view(np.uint16) if bd > 8 else u).reshape(h, w)
+    if black_level_ is not None:
+        u = black_level(u, pattern, bd, black_level_, per_channel)
+    if lens is not None:
+        u = lens_shading(u, pattern, bd, lens[0], lens[1])
+    e, rgb = demosaic(u.view(np.uint8).ravel(), w, h, pattern, algo, bd, False)
+    if e != 0:
+        raise RuntimeError(f"oracle demosaic returned {e}")
+    if epilogue is not None and epilogue[0]:
+        rgb = isp_epilogue(rgb, bd, *epilogue)
+    return rgb
+
+
+def ref_pipeline(raw: np.ndarray, w: int, h: int, pattern: int, algo: int, bd: int, packed: bool = False,
+                 black_level_=None, per_channel: bool = False, lens=None, epilogue=None) -> np.ndarray:
+    """`pipeline` composed from the compiled reference's own functions."""
+    lib = reference()
+    raw = np.ascontiguousarray(raw).view(np.uint8).ravel()
+    if packed:
+        u = np.zeros(w * h * (2 if bd > 8 else 1), dtype=np.uint8)
+        lib.ref_unpack(_ptr(raw.copy()), w, h, bd, int(packed), _ptr(u))
+    else:
+        u = raw[: w * h * (2 if bd > 8 else 1)].copy()
+    if black_level_ is not None:
+        bl = np.asarray(black_level_, dtype=np.float32).ravel()
+        bl = np.ascontiguousarray(np.repeat(bl, 4) if bl.size == 1 else bl, dtype=np.float32)
+        e = lib.ref_black_level(_ptr(u), w, h, pattern, bd, bl.ctypes.data_as(_fp), int(per_channel))
+        if e != 0:
+            raise RuntimeError(f"reference black level returned {e}")
+    img = u.view(np.uint16 if bd > 8 else np.uint8).reshape(h, w)
+    if lens is not None:
+        img = ref_lens_shading(img, pattern, bd, lens[0], lens[1])
+    out = np.zeros(rgb_bytes(w, h, bd), dtype=np.uint8)
+    e = lib.ref_demosaic_cpu(_ptr(np.ascontiguousarray(img).view(np.uint8).ravel().copy()), _ptr(out), w, h,
+                             pattern, algo, bd, 0)
+    if e != 0:
+        raise RuntimeError(f"reference demosaic returned {e}")
+    rgb = _as_rgb(out, w, h, bd)
+    if epilogue is not None:
+        flags, gains, m, gamma = epilogue
+        if flags & 1:
+            rgb = ref_wb(rgb, bd, gains)
+        if flags & 2:
+            rgb = ref_ccm(rgb, bd, m)
+        if flags & 4:
+            rgb = ref_gamma(rgb, bd, gamma)
+    return rgb
+
+
 # ------------------------------------------------------------ FP64 ISP chain
 
 _dp = ctypes.POINTER(ctypes.c_double)

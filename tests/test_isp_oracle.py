@@ -166,6 +166,59 @@ def test_lens_shading_planted_mistake_is_detected():
     np.testing.assert_array_equal(O.lens_shading(img, 0, bd, coef), ref)
 
 
+# ------------------------------------------------------------ whole single-frame path
+
+PIPE_CASES = [  # (w, h, bd, packed, pattern, algo, bl, lens case, epilogue flags, above max)
+    (38, 24, 12, True, 0, O.MG, [64.0] * 4, 0, 7, False),
+    (40, 22, 10, True, 1, O.HQLI, [16.0, 20.0, 24.0, 28.0], 1, 4, False),
+    (36, 20, 12, False, 2, O.MG, None, 4, 3, True),     # u16 samples above the maximum reach the lens stage
+    (34, 26, 8, False, 3, O.PRISM, [8.0] * 4, 0, 7, False),
+    (30, 18, 16, False, 0, O.RCD, [300.0] * 4, 1, 0, False),
+    (42, 20, 14, True, 2, O.AHD, None, 0, 2, False),
+]
+
+
+def _pipe_raw(case, seed):
+    w, h, bd, packed, pattern, algo, bl, ci, flags, above = case
+    rng = np.random.default_rng(seed)
+    if above:
+        vals = rng.integers(0, 65536, size=(h, w)).astype(np.uint16)
+        return vals.view(np.uint8).ravel()
+    vals = rng.integers(0, 1 << bd, size=(h, w))
+    from imop_amd import scene
+    return scene.raw_bytes(vals.astype(np.uint16 if bd > 8 else np.uint8), bd, packed)
+
+
+@pytest.mark.skipif(not (O.reference_available() and O.reference_isp_available()),
+                    reason="oracle/_ref not built (reference absent)")
+@pytest.mark.parametrize("k", range(len(PIPE_CASES)))
+def test_pipeline_matches_reference_composition(k):
+    """unpack -> black level -> lens shading -> demosaic -> epilogue: the
+    oracle's composition equals the reference's functions run in that order."""
+    case = PIPE_CASES[k]
+    w, h, bd, packed, pattern, algo, bl, ci, flags, _ = case
+    raw = _pipe_raw(case, 40 + k)
+    epi = (flags, GAINS[1], MATRICES[0], 2.2)
+    got = O.pipeline(raw, w, h, pattern, algo, bd, packed, bl, True, LS_CASES[ci], epi)
+    want = O.ref_pipeline(raw, w, h, pattern, algo, bd, packed, bl, True, LS_CASES[ci], epi)
+    np.testing.assert_array_equal(got, want)
+    # the lens stage is not vacuous here
+    assert not np.array_equal(got, O.pipeline(raw, w, h, pattern, algo, bd, packed, bl, True, None, epi))
+
+
+def test_pipeline_without_lens_is_the_demosaic_and_epilogue():
+    w, h, bd = 38, 24, 12
+    raw = _pipe_raw((w, h, bd, True, 0, O.MG, None, 0, 7, False), 5)
+    epi = (7, GAINS[1], MATRICES[0], 2.2)
+    e, rgb = O.demosaic(raw, w, h, 0, O.MG, bd, True, [64.0] * 4, True)
+    assert e == 0
+    np.testing.assert_array_equal(O.pipeline(raw, w, h, 0, O.MG, bd, True, [64.0] * 4, True, None, epi),
+                                  O.isp_epilogue(rgb, bd, *epi))
+    # zero coefficients: the lens stage is the identity
+    np.testing.assert_array_equal(O.pipeline(raw, w, h, 0, O.MG, bd, True, [64.0] * 4, True, LS_CASES[2][:1] +
+                                             ((0.5, 0.5),), epi), O.isp_epilogue(rgb, bd, *epi))
+
+
 # ------------------------------------------------------------ defect correction
 
 def _defect_img(kind, bd, h, w, seed):
