@@ -75,51 +75,148 @@ TRAFFIC_PROFILES = {
 }
 
 
-def measured_traffic(name):
-    """(bytes per launch, source file) from the committed rocprofv3 summary, or (None, None)."""
-    rel = TRAFFIC_PROFILES.get(name)
+def _profile_value(rel, key):
     if rel is None:
-        return None, None
+        return None
     try:
         with open(os.path.join(ROOT, rel)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
-                    return int(line.split("=")[1]), rel
+                if line.startswith(key):
+                    return int(line.split("=")[1])
     except OSError:
         pass
-    return None, None
+    return None
 
 
-def cpu_baseline(raw, name, budget_s=12.0):
+def measured_traffic(name):
+    """(bytes per launch, source file) from the committed rocprofv3 summary, or (None, None)."""
+    rel = TRAFFIC_PROFILES.get(name)
+    v = _profile_value(rel, "traffic_bytes_per_launch")
+    return (v, rel) if v is not None else (None, None)
+
+
+# Kernels bound by vector-ALU issue rather than HBM: rocprofv3 summaries
+# holding SQ_INSTS_VALU per frame (wave64 instructions, every kernel of the
+# frame summed).  The VALU roofline: each wave64 instruction occupies a
+# 16-lane SIMD for 4 cycles, so the part issues at most 1,024 SIMDs x 16
+# lanes x 2.4 GHz = 39.3 T lane-operations per second.
+VALU_PROFILES = {
+    "prism10": "profiles/prism10_trace_traffic_counters.md",
+    "rcd10": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "sf12": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "l7_12": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "ahd12": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "amaze12": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "dfpd12": "profiles/demosaic_algorithms_12mp_valu_counters.md",
+    "lens12": "profiles/k_lens_shading_12mp_u16_trace_traffic.md",
+    "chain8": "profiles/chain_8x1006x758_trace_and_traffic.md",
+}
+VALU_PEAK_LANE_OPS = 1024 * 16 * 2.4e9
+
+
+def binding_roofline(hbm, valu):
+    """{"roofline": the bound the kernel is closest to, "<other>_roofline":
+    the other one} -- a kernel is bound by the resource it uses the larger
+    fraction of."""
+    if valu is None:
+        return {"roofline": hbm}
+    if valu["frac"] >= hbm["frac"]:
+        return {"roofline": valu, "hbm_roofline": hbm}
+    return {"roofline": hbm, "valu_roofline": valu}
+
+
+def valu_roofline(name, seconds_per_frame, key=None):
+    """The VALU roofline object of a workload from its committed SQ counters,
+    or None when no summary holds them."""
+    rel = VALU_PROFILES.get(name)
+    insts = _profile_value(rel, key or f"valu_wave_insts_per_frame[{name}]")
+    if insts is None:
+        return None
+    achieved = insts * 64 / seconds_per_frame
+    return {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
+            "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK_LANE_OPS, 4),
+            "valu_wave_insts_per_frame": insts, "counter_source": rel}
+
+
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo), or the platform's name."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def _timed_frames(fn, budget_s, warmup, min_iters, max_iters=200):
+    """The reference harness's timing method (benchmarks/benchmark.cpp:36-50,
+    the run loop and trimming of its CPU leg): warmup untimed calls, then
+    calls until min_iters are done and budget_s has passed (at most
+    max_iters); samples outside [Q1 - 1.5 IQR, Q3 + 1.5 IQR] are dropped
+    and the median of the rest is the frame time.  Returns (median s, kept
+    samples, dropped samples)."""
+    for _ in range(warmup):
+        fn()
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < min_iters or (time.perf_counter() < t_end and len(times) < max_iters):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    n = len(times)
+    q1, q3 = times[n // 4], times[(3 * n) // 4]
+    lo, hi = q1 - 1.5 * (q3 - q1), q3 + 1.5 * (q3 - q1)
+    kept = [t for t in times if lo <= t <= hi] or times
+    return kept[len(kept) // 2], len(kept), n - len(kept)
+
+
+def cpu_baseline(raw, name, budget_s=2.0, single_budget_s=4.0):
     """The reference CPU path on this host's cores: unpack (pixel::get_raw) ->
-    black_level::process_global -> imop::demosaic_cpu, as one frame of work;
-    repeated for about budget_s seconds, median frame time."""
+    black_level::process_global -> imop::demosaic_cpu, as one frame of work,
+    timed as the reference harness times its CPU leg (2 warm-ups, at least 5
+    frames, a 2 s budget, IQR-trimmed median) with OpenMP over every core the
+    process may use, and again on one thread (1 warm-up, at least 2 frames)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     w, h, bd, packed, algo, bl, seed, lines = WORKLOADS[name]
     blv = None if bl is None else [bl] * 4
     if O.reference_available():
-        fn, kind, threads = O.ref_demosaic, "reference", O.reference().ref_num_threads()
+        lib, fn, kind = O.reference(), O.ref_demosaic, "reference"
+        get_threads, set_threads = lib.ref_num_threads, lib.ref_set_num_threads
     else:
-        fn, kind, threads = O.demosaic, "port", O.oracle().orc_num_threads()
-    times = []
-    t_end = time.time() + budget_s
-    while len(times) < 3 or (time.time() < t_end and len(times) < 200):
-        t0 = time.perf_counter()
+        lib, fn, kind = O.oracle(), O.demosaic, "port"
+        get_threads, set_threads = lib.orc_num_threads, lib.orc_set_num_threads
+
+    def frame():
         e, _ = fn(raw, w, h, 0, algo, bd, packed, blv)
-        times.append(time.perf_counter() - t0)
         if e != 0:
             raise RuntimeError(f"CPU baseline returned {e}")
-    times.sort()
-    med = times[len(times) // 2]
+
+    threads = int(get_threads())
+    med, kept, dropped = _timed_frames(frame, budget_s, 2, 5)
+    set_threads(1)
+    try:
+        med1, kept1, _ = _timed_frames(frame, single_budget_s, 1, 2)
+    finally:
+        set_threads(threads)
     return {
         "value": round(w * h / med / 1e6, 2),
         "unit": "Mpix/s",
-        "cores": int(threads),
+        "cores": threads,
         "kind": kind,
-        "sample": f"{len(times)} full {w}x{h} frames of the same workload, median frame {med * 1e3:.1f} ms "
-                  f"(OpenMP rows, {threads} threads; unpack and black level single-threaded as in the reference)",
+        "cpu_model": cpu_model(),
+        "ms_per_frame": round(med * 1e3, 3),
+        "single_thread": {"value": round(w * h / med1 / 1e6, 2), "unit": "Mpix/s", "cores": 1,
+                          "ms_per_frame": round(med1 * 1e3, 3), "frames": kept1},
+        "sample": f"full {w}x{h} frames of the same workload: 2 warm-ups, {kept + dropped} timed "
+                  f"({dropped} outside 1.5 IQR dropped), median {med * 1e3:.1f} ms with {threads} OpenMP threads "
+                  f"(row loops; unpack and black level single-threaded as in the reference); "
+                  f"one thread: {kept1} frames, median {med1 * 1e3:.1f} ms",
     }
 
 
@@ -262,7 +359,7 @@ def bench_batch(steps, warmup, with_cpu, shape=None):
     return out
 
 
-def bench_single(name, steps, warmup, sets=4, graph=False):
+def bench_single(name, steps, warmup, sets=4, graph=False, with_cpu=False):
     """One single-frame workload of WORKLOADS as a secondary measurement:
     parity against the reference digest, then back-to-back launches over
     rotating buffer sets.  graph: the launches are captured once into a HIP
@@ -330,11 +427,15 @@ def bench_single(name, steps, warmup, sets=4, graph=False):
     ms = a.elapsed_time(b) / steps
     alg = int(raw.size) + imop_amd.rgb_byte_size(w, h, bd)
     log(f"bench: {name} {ms * 1e3:.1f} us per frame, parity {parity}")
-    return {"workload": name, "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s",
-            "ms_per_frame": round(ms, 5), "parity": parity, "timing": "graph replay" if graph else "eager launches",
-            "roofline": {"bound": "hbm", "algorithmic_bytes_per_frame": alg,
-                         "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    hbm = {"bound": "hbm", "algorithmic_bytes_per_frame": alg, "achieved": round(alg / (ms / 1e3) / 1e9, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"workload": name, "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+           "ms_per_frame": round(ms, 5), "parity": parity, "timing": "graph replay" if graph else "eager launches"}
+    out.update(binding_roofline(hbm, valu_roofline(name, ms / 1e3)))
+    if with_cpu:
+        log(f"bench: timing the CPU baseline of {name}")
+        out["cpu_baseline"] = cpu_baseline(raw, name)
+    return out
 
 
 # Epilogue of the ISP measurement: white balance gains, colour matrix, gamma
@@ -527,6 +628,7 @@ def bench_lens_shading(steps, warmup, with_cpu, sets=6):
                         "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
            "steps": steps, "warmup": warmup}
+    out.update(binding_roofline(out["roofline"], valu_roofline("lens12", ms / 1e3)))
     if with_cpu:
         fn, kind = (O.ref_lens_shading, "reference") if O.reference_isp_available() else (O.lens_shading, "port")
         times = []
@@ -1058,6 +1160,10 @@ def bench_chain(steps, warmup, with_cpu):
         "steps": steps,
         "warmup": warmup,
     }
+    vf = valu_roofline("chain8", ms_f / 1e3, "valu_wave_insts_per_step[chain8_fwd]")
+    vb = valu_roofline("chain8", ms_b / 1e3, "valu_wave_insts_per_step[chain8_bwd]")
+    if vf is not None and vb is not None:
+        out["valu_roofline"] = {"forward_loss": vf, "backward": vb}
     if with_cpu:
         out["cpu_baseline"], out["gradient_check"] = chain_cpu_leg(raw, target)
     return out
@@ -1385,14 +1491,15 @@ def main():
         result["batched"] = bench_batch(args.batch_steps, 3, not args.no_cpu_baseline)
     if not args.no_batch and args.workload == "mg12":
         torch.cuda.empty_cache()
-        result["prism"] = bench_single("prism10", 20, 3)
-        result["rcd"] = bench_single("rcd10", 20, 3)
-        result["super_fast"] = bench_single("sf12", 50, 5)
-        result["l7"] = bench_single("l7_12", 50, 5)
-        result["ahd"] = bench_single("ahd12", 20, 3)
-        result["amaze"] = bench_single("amaze12", 50, 5)
-        result["dfpd"] = bench_single("dfpd12", 50, 5)
-        result["hqli_small"] = bench_single("hqli8", 200, 20, graph=True)
+        cpu = not args.no_cpu_baseline
+        result["prism"] = bench_single("prism10", 20, 3, with_cpu=cpu)
+        result["rcd"] = bench_single("rcd10", 20, 3, with_cpu=cpu)
+        result["super_fast"] = bench_single("sf12", 50, 5, with_cpu=cpu)
+        result["l7"] = bench_single("l7_12", 50, 5, with_cpu=cpu)
+        result["ahd"] = bench_single("ahd12", 20, 3, with_cpu=cpu)
+        result["amaze"] = bench_single("amaze12", 50, 5, with_cpu=cpu)
+        result["dfpd"] = bench_single("dfpd12", 50, 5, with_cpu=cpu)
+        result["hqli_small"] = bench_single("hqli8", 200, 20, graph=True, with_cpu=cpu)
         result["mg_gige10"] = bench_single("mg10g", 50, 5)
     if not args.no_chain and args.workload == "mg12":
         torch.cuda.empty_cache()
