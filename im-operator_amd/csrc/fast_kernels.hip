@@ -13,11 +13,13 @@ namespace imop_hip {
 // rows | wpb << 16 | xcd_remap << 24.  rows: rows per wave (2 or 4; 0 =
 // automatic); wpb: waves per group (2 or 4).  xcd_remap bit 0:
 // XCD-contiguous group order; bit 1: strip-fastest group order.  Automatic
-// rows: 4, except that a frame that would launch fewer than kSmallFrameWaves
-// waves with 4-row waves (under 4 per SIMD) uses 2, twice the waves to cover
-// the latency of a small frame.
+// rows: 2.  Groups of 4 waves x 2 rows (8 output rows) decode 14 raw rows
+// where 4 x 4 groups decode 22 for 16, but the kernel is bound by its RGB
+// stores, and 8-row groups keep fewer rows of the frame in flight at once:
+// 12 MP MG frame (RAW12 -> u16) 16.2-17.1 us against 17.1-17.9 us for 4-row
+// waves on the same boxes; small frames gain the same way from twice the
+// waves.
 static std::atomic<uint32_t> g_shape{0u | (4u << 16) | (3u << 24)};
-constexpr long kSmallFrameWaves = 4096;
 
 void set_launch_shape(int rows, int wpb, int xcd_remap) {
     const bool rows_ok = rows == 0 || rows == 2 || rows == 4;
@@ -97,7 +99,7 @@ hipError_t launch_fast(const FrameParams& f, int mode, hipStream_t st, const Epi
         rows = 4;
         wpb = 4;
     } else if (rows == 0) {
-        rows = (long)strip_count(f.w, f.out16 != 0) * ((f.h + 3) / 4) < kSmallFrameWaves ? 2 : 4;
+        rows = 2;
     }
     FastParams q = make_fast(f, rows, wpb, xcd);
     if (epi && epi->flags) q.epi = *epi;

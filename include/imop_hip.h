@@ -104,16 +104,16 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
                               const float* offsets);
 
 /* Launch shape of the register-resident HQLI / MG kernel, chosen by the
- * autotuner (im-operator_amd/imop_amd/autotune.py): rows of a frame each
- * wave walks down (2, 4, 8 or 16), waves per workgroup (1, 2 or 4), and the
- * wave order: bit 0 remaps workgroups so each XCD works on a contiguous run
- * of waves (else launch order), bit 1 orders waves strip-fastest (adjacent
- * waves cover adjacent columns of the same rows; else they walk down a
- * strip).  Out-of-range values (e.g. -1) leave a setting
- * unchanged.  rows 0 is the automatic band height, the initial setting: 4
- * rows, or 2 for a frame that would launch fewer than 4096 waves with 4
- * (small frames; the fused-epilogue form keeps 4); get reports 0 while it is
- * in effect, so a get / set round trip keeps it.  Process-wide; has no counterpart in the reference (its CUDA
+ * autotuner (im-operator_amd/imop_amd/autotune.py): rows each wave
+ * computes (2 or 4), waves per workgroup stacked down one strip (2 or 4),
+ * and the workgroup order: bit 0 remaps workgroups so each XCD works on a
+ * contiguous run of them (else launch order), bit 1 orders them
+ * strip-fastest (adjacent workgroups cover adjacent columns of the same
+ * rows; else they walk down a strip).  Out-of-range values (e.g. -1) leave
+ * a setting unchanged.  rows 0 is the automatic band height, the initial
+ * setting: 2 rows (the fused-epilogue form keeps 4); get reports 0 while it
+ * is in effect, so a get / set round trip keeps it.  The initial shape is
+ * (0, 4, 3).  Process-wide; has no counterpart in the reference (its CUDA
  * launch shape is fixed at 32x16, src/algorithms/cuda_kernels.cu:1496). */
 int imop_hip_set_launch_shape(int rows_per_wave, int waves_per_block, int xcd_remap);
 int imop_hip_get_launch_shape(int* rows_per_wave, int* waves_per_block, int* xcd_remap);
