@@ -888,8 +888,12 @@ __device__ __forceinline__ void d_from_own(int py, int own0, int own1, int D[3])
     }
 }
 
-// One wave W (compile time) of an interior group.
-template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, bool LENS, int W>
+// One wave W (compile time) of a group whose rows lie inside the frame's
+// vertical rim.  HC: the group's strip reaches the left or right rim, so
+// loads and samples of lanes outside the frame are masked, colour
+// differences take the sentinels of the columns the pass never reaches,
+// and the output takes the border column fill (band_step's careful form).
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, bool LENS, int W, bool HC = false>
 __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp, int yg, uint8_t* L,
                                            const uint16_t* lut) {
     using G = GroupLayout<TH, GB>;
@@ -907,12 +911,13 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
     {
         Raw raw[NI];
 #pragma unroll
-        for (int k = 0; k < NI; ++k) raw[k] = load_row<FMT, false, ODD>(f, yg - 3 + W + GB * k, xu, lane_ok, rs, lo);
+        for (int k = 0; k < NI; ++k)
+            raw[k] = load_row<FMT, HC, ODD, false>(f, yg - 3 + W + GB * k, xu, lane_ok, rs, lo);
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
             const int i = W + GB * k;
             int P[4];
-            decode_row<FMT, false, ODD, YODD, false, LENS>(f, x, yg - 3 + i, raw[k], lane_ok, xu, P);
+            decode_row<FMT, HC, ODD, YODD, false, LENS>(f, x, yg - 3 + i, raw[k], lane_ok, xu, P);
             Lw[i * 128 + lane] = (uint32_t)P[0] | ((uint32_t)P[2] << 16);
             Lw[i * 128 + 64 + lane] = (uint32_t)P[1] | ((uint32_t)P[3] << 16);
         }
@@ -941,7 +946,7 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
                 }
             }
             int D[3];
-            d_row<PAT, MODE, false>(P5, PY, 1, col_ok, D);
+            d_row<PAT, MODE, HC>(P5, PY, 1, col_ok, D);
             Lw[G::kDOff / 4 + j * 128 + lane] = (uint32_t)(PC == 0 ? D[0] : D[1]);
             Lw[G::kDOff / 4 + j * 128 + 64 + lane] = (uint32_t)(PC == 0 ? D[1] : D[2]);
         }
@@ -965,7 +970,7 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
     else stage = Lw + G::kDOff / 4 + W * 384;            // HQLI: no colour-difference rows
     // MG into u16 RGB without the epilogue: interpolation and whole-chunk
     // staged stores inline; every other form through band_step
-    constexpr bool lean = MODE != MODE_HQLI && FMT != FMT_U8 && !EPI;
+    constexpr bool lean = MODE != MODE_HQLI && FMT != FMT_U8 && !EPI && !HC;
     const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
 #pragma unroll
     for (int s = 0; s < TH / 2; ++s) {
@@ -1003,8 +1008,8 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
                                     h0 - sp.xs, R[a], Gc[a], B[a]);
             }
         } else {
-            band_step<FMT, PAT, MODE, false, ODD, EPI, true, FMT == FMT_U8 ? 2 : 1>(f, sp, x, yA, stage, Pw, Dw,
-                                                                                 col_ok, col_core, lut);
+            band_step<FMT, PAT, MODE, HC, ODD, EPI, true, FMT == FMT_U8 ? 2 : 1>(f, sp, x, yA, stage, Pw, Dw,
+                                                                              col_ok, col_core, lut);
         }
     }
 }
@@ -1014,23 +1019,40 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
 // Group -> (strip, group row).  Blocks are spread round-robin over the 8
 // XCDs; the remap (xcd_remap bit 0) gives each XCD a contiguous run of
 // groups, so the raw rows that vertically adjacent groups share are read
-// through one L2.  Strip-fastest (bit 1): consecutive groups cover a group
-// row left to right; else they walk down a strip.  The quotient uses the
-// host's reciprocal place_mul = ceil(2^40 / divisor), exact while block count
-// x divisor < 2^40 (the host leaves place_mul 0 otherwise: plain division).
+// through one L2.  XCDs 4-7 take their runs in reverse, so the frame's last
+// rows, whose groups run the slower per-wave rim path, start first instead
+// of finishing the launch (12 MP MG frame: 16.1 -> 15.8 us).  Strip-fastest
+// (bit 1): consecutive groups cover a group row left to right; else they
+// walk down a strip.  The quotient uses the host's reciprocal place_mul =
+// ceil(2^40 / divisor), exact while block count x divisor < 2^40 (the host
+// leaves place_mul 0 otherwise: plain division).
 __device__ __forceinline__ void group_place(const FastParams& f, int& sx, int& g) {
     const int nblk = gridDim.x;
-    // with nblk = 8q + r, XCD x runs q + (x < r) blocks: a bijection of [0, nblk)
+    // with nblk = 8q + r, XCD x runs cnt = q + (x < r) blocks: a bijection of [0, nblk)
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int q8 = nblk >> 3, r8 = nblk & 7;
+    const int cnt = q8 + (xcd < r8 ? 1 : 0);
+    const int run = xcd >= 4 ? cnt - 1 - slot : slot;
     const uint32_t lblk = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + slot : (int)blockIdx.x);
+        (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + run : (int)blockIdx.x);
     const bool strip_fastest = (f.xcd_remap & 2) != 0;
     const uint32_t d = strip_fastest ? (uint32_t)f.nsx : (uint32_t)f.nby;
     const uint32_t q = f.place_mul ? (uint32_t)(((uint64_t)lblk * f.place_mul) >> 40) : lblk / d;
     const uint32_t r = lblk - q * d;
     g = (int)(strip_fastest ? q : r);
     sx = (int)(strip_fastest ? r : q);
+}
+
+// Wave w (wave-uniform) of a group on the shared-LDS path.
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, bool LENS, bool HC>
+__device__ __forceinline__ void group_waves(int w, const FastParams& f, const Strip& sp, int yg, uint8_t* L,
+                                            const uint16_t* lut) {
+    switch (w) {
+    case 0: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 0, HC>(f, sp, yg, L, lut); break;
+    case 1: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 1, HC>(f, sp, yg, L, lut); break;
+    case 2: if (GB > 2) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 2 ? 2 : 0), HC>(f, sp, yg, L, lut); break;
+    default: if (GB > 3) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 3 ? 3 : 0), HC>(f, sp, yg, L, lut); break;
+    }
 }
 
 template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI = false, bool LENS = false>
@@ -1050,29 +1072,26 @@ __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
     const Strip sp = strip_of(f, sx);
     const int yg = g * G::kRows;
     uint8_t* L = (uint8_t*)s_group;
-    // rim: the group's outputs touch the 3-pixel frame rim, or its raw
-    // window reaches the stream's final row.  Its waves take their bands one
-    // by one on the per-wave path (vrim: the band's rows reach the rim, else
-    // only the strip's columns do).
+    // vertical rim: the group's outputs touch the frame's top or bottom
+    // 3-pixel rim, or its raw window reaches the stream's final row.  Its
+    // waves take their bands one by one on the per-wave path (vrim: the
+    // band's rows reach the rim, else only the strip's columns do).  Groups
+    // whose strip reaches the left or right rim run the group path in its
+    // careful-column form.
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const bool rim = yg < 3 || yg + G::kRows + 3 > f.h - 1 || sp.xs < 3 || sp.xs + 248 > f.w - 3;
-    if (rim) {
+    const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
+    if (yg < 3 || yg + G::kRows + 3 > f.h - 1) {
         const int y0 = yg + TH * w;
         if (y0 >= f.h) return;
         uint32_t* stage = (uint32_t*)L + w * 384;  // 1536 bytes of the group's LDS per wave
         const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
-        const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
         if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true, LENS>(f, sp, y0, stage, s_lut);
         else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
         else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
         return;
     }
-    switch (w) {
-    case 0: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 0>(f, sp, yg, L, s_lut); break;
-    case 1: group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, 1>(f, sp, yg, L, s_lut); break;
-    case 2: if (GB > 2) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 2 ? 2 : 0)>(f, sp, yg, L, s_lut); break;
-    default: if (GB > 3) group_wave<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, (GB > 3 ? 3 : 0)>(f, sp, yg, L, s_lut); break;
-    }
+    if (hrim) group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, true>(w, f, sp, yg, L, s_lut);
+    else group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, false>(w, f, sp, yg, L, s_lut);
 }
 
 // ---------------------------------------------------------------- launchers
