@@ -378,7 +378,20 @@ __global__ void __launch_bounds__(256) k_defect_pass(DefectArgs a) {
     }
 }
 
+// After `passes` passes: *status = 0 when the last pass changed nothing (the
+// frame is the fixed point), 1 when it still changed pixels, 2 when a tile
+// reached its local iteration cap.
+__global__ void k_defect_status(const int* ctl, int passes, int* status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *status = ctl[3] ? 2 : (ctl[(passes - 1) % 3] == 0 ? 0 : 1);
+}
+
 }  // namespace
+
+hipError_t launch_defect_status(const int* ctl, int passes, int* status, hipStream_t st) {
+    hipLaunchKernelGGL(k_defect_status, dim3(1), dim3(64), 0, st, ctl, passes, status);
+    return hipGetLastError();
+}
 
 hipError_t launch_defect_copy(const uint8_t* src, uint8_t* dst, uint64_t nbytes, int* ctl, hipStream_t st) {
     const bool vec = (((uintptr_t)src | (uintptr_t)dst) & 15u) == 0;

@@ -100,6 +100,7 @@ hipError_t launch_bayer_prologue(const FrameParams& f, const LensParams& lens, u
 hipError_t launch_defect_copy(const uint8_t* src, uint8_t* dst, uint64_t nbytes, int* ctl, hipStream_t st);
 hipError_t launch_defect_pass(uint8_t* data, const uint8_t* orig, uint8_t* flags, int* ctl, int w, int h, int bd,
                               int thresh, int pass, hipStream_t st);
+hipError_t launch_defect_status(const int* ctl, int passes, int* status, hipStream_t st);
 int defect_tiles_x(int w);
 int defect_tiles_y(int h);
 

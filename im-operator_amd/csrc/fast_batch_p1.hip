@@ -2,7 +2,8 @@
 #include "fast_batch_impl.hpp"
 
 namespace imop_hip {
-hipError_t launch_batch_pattern1(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st) {
-    return launch_batch_p<1>(a, alg, rows, blocks, st);
+hipError_t launch_batch_pattern1(const BatchArgs& a, int alg, int fmt, bool odd, int rows, bool persist,
+                                  int blocks, hipStream_t st) {
+    return launch_batch_p<1>(a, alg, fmt, odd, rows, persist, blocks, st);
 }
 }  // namespace imop_hip

@@ -859,7 +859,10 @@ __device__ __forceinline__ void lds_sync() { __syncthreads(); }
 // for 2-row waves, separate 1536-byte staging slices.  With 4-row waves a
 // wave's staging slice aliases LDS no other wave reads in phase 3: its own
 // decoded rows under MG (read into registers first), the colour-difference
-// rows under HQLI (which has none).
+// rows under HQLI (which has none).  The staged-row reads of
+// store_row_u16_whole reach up to 2,191 bytes past a slice's start (a
+// 144-byte window offset plus two 1 KB chunk rows), so the separate slices
+// are followed by 768 bytes that keep the last wave's reads inside the group.
 template <int TH, int GB>
 struct GroupLayout {
     static constexpr int kRows = TH * GB;
@@ -868,7 +871,7 @@ struct GroupLayout {
     static constexpr int kDOff = kPR * 512;
     static constexpr int kSOff = kDOff + kDR * 512;
     static constexpr bool kSepStage = TH < 4;
-    static constexpr int kBytes = kSOff + (kSepStage ? GB * 1536 : 0);
+    static constexpr int kBytes = kSOff + (kSepStage ? GB * 1536 + 768 : 0);
 };
 
 __device__ __forceinline__ void unpack4(uint32_t X, uint32_t Y, int p[4]) {
@@ -1055,6 +1058,33 @@ __device__ __forceinline__ void group_waves(int w, const FastParams& f, const St
     }
 }
 
+// One workgroup item (of the single-frame kernel or of the batched one): the
+// group row from yg of strip sp, as wave w (wave-uniform).  Vertical rim:
+// the group's outputs touch the frame's top or bottom 3-pixel rim, or its
+// raw window reaches the stream's final row; its waves take their bands one
+// by one on the per-wave path (vrim: the band's rows reach the rim, else
+// only the strip's columns do).  Groups whose strip reaches the left or
+// right rim run the shared path in its careful-column form.  L holds
+// GroupLayout<TH, GB>::kBytes.  No barrier is left pending on return.
+template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI, bool LENS>
+__device__ __forceinline__ void group_item(const FastParams& f, const Strip& sp, int yg, int w, uint8_t* L,
+                                           const uint16_t* lut) {
+    using G = GroupLayout<TH, GB>;
+    const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
+    if (yg < 3 || yg + G::kRows + 3 > f.h - 1) {
+        const int y0 = yg + TH * w;
+        if (y0 >= f.h) return;
+        uint32_t* stage = (uint32_t*)L + w * 384;  // 1536 bytes of the group's LDS per wave
+        const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
+        if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true, LENS>(f, sp, y0, stage, lut);
+        else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, lut);
+        else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, lut);
+        return;
+    }
+    if (hrim) group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, true>(w, f, sp, yg, L, lut);
+    else group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, false>(w, f, sp, yg, L, lut);
+}
+
 template <int FMT, int PAT, int MODE, int TH, int GB, bool ODD, bool EPI = false, bool LENS = false>
 __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
     using G = GroupLayout<TH, GB>;
@@ -1069,36 +1099,16 @@ __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
     }
     int sx, g;
     group_place(f, sx, g);
-    const Strip sp = strip_of(f, sx);
-    const int yg = g * G::kRows;
-    uint8_t* L = (uint8_t*)s_group;
-    // vertical rim: the group's outputs touch the frame's top or bottom
-    // 3-pixel rim, or its raw window reaches the stream's final row.  Its
-    // waves take their bands one by one on the per-wave path (vrim: the
-    // band's rows reach the rim, else only the strip's columns do).  Groups
-    // whose strip reaches the left or right rim run the group path in its
-    // careful-column form.
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
-    if (yg < 3 || yg + G::kRows + 3 > f.h - 1) {
-        const int y0 = yg + TH * w;
-        if (y0 >= f.h) return;
-        uint32_t* stage = (uint32_t*)L + w * 384;  // 1536 bytes of the group's LDS per wave
-        const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
-        if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, true, LENS>(f, sp, y0, stage, s_lut);
-        else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
-        else run_band<FMT, PAT, MODE, false, TH, ODD, EPI, false, LENS>(f, sp, y0, stage, s_lut);
-        return;
-    }
-    if (hrim) group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, true>(w, f, sp, yg, L, s_lut);
-    else group_waves<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS, false>(w, f, sp, yg, L, s_lut);
+    group_item<FMT, PAT, MODE, TH, GB, ODD, EPI, LENS>(f, strip_of(f, sx), g * G::kRows, w, (uint8_t*)s_group,
+                                                        s_lut);
 }
 
 // ---------------------------------------------------------------- launchers
 
 template <int FMT, int PAT, int MODE, bool ODD>
 static void launch_fmpo(const FastParams& f, hipStream_t st) {
-    // f.nsx strips x f.nby groups, f.wpb waves per group, f.th rows per wave
+    // f.nsx strips x f.nby groups of 4 waves x f.th rows
     const int blocks = f.nsx * f.nby;
     // the ISP forms (RGB epilogue, with or without the lens prologue): 4-row waves in groups of 4
     if (f.lens.on) {
@@ -1109,13 +1119,8 @@ static void launch_fmpo(const FastParams& f, hipStream_t st) {
         hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD, true>), dim3(blocks), dim3(256), 0, st, f);
         return;
     }
-    if (f.th == 2) {
-        if (f.wpb == 2) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, 2, ODD>), dim3(blocks), dim3(128), 0, st, f);
-        else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
-    } else {
-        if (f.wpb == 2) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 2, ODD>), dim3(blocks), dim3(128), 0, st, f);
-        else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
-    }
+    if (f.th == 2) hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 2, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
+    else hipLaunchKernelGGL((k_fast<FMT, PAT, MODE, 4, 4, ODD>), dim3(blocks), dim3(256), 0, st, f);
 }
 
 template <int FMT, int PAT, int MODE>
@@ -1125,12 +1130,16 @@ static hipError_t launch_fmp(const FastParams& f, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The reference's 8-bit integer MG runs on 8-bit samples demosaiced from an
+// unpacked buffer only, so only the u8 layout instantiates it.
 template <int FMT, int PAT>
 static hipError_t launch_fp(const FastParams& f, int mode, hipStream_t st) {
     switch (mode) {
     case MODE_HQLI: return launch_fmp<FMT, PAT, MODE_HQLI>(f, st);
     case MODE_MG_EXACT: return launch_fmp<FMT, PAT, MODE_MG_EXACT>(f, st);
-    default: return launch_fmp<FMT, PAT, MODE_MG_INT8>(f, st);
+    default:
+        if constexpr (FMT == FMT_U8) return launch_fmp<FMT, PAT, MODE_MG_INT8>(f, st);
+        else return hipErrorInvalidValue;
     }
 }
 

@@ -11,7 +11,8 @@ namespace imop_hip {
 // Launch shape (set by the autotuner through imop_hip_set_launch_shape), one
 // word so that a launch never sees half of a concurrent change:
 // rows | wpb << 16 | xcd_remap << 24.  rows: rows per wave (2 or 4; 0 =
-// automatic); wpb: waves per group (2 or 4).  xcd_remap bit 0:
+// automatic); wpb: waves per group (4; groups of 2 waves measured no faster
+// and are not instantiated).  xcd_remap bit 0:
 // XCD-contiguous group order; bit 1: strip-fastest group order.  Automatic
 // rows: 2.  Groups of 4 waves x 2 rows (8 output rows) decode 14 raw rows
 // where 4 x 4 groups decode 22 for 16, but the kernel is bound by its RGB
@@ -27,7 +28,7 @@ void set_launch_shape(int rows, int wpb, int xcd_remap) {
     for (;;) {
         int r = (int)(cur & 0xFFFFu), w = (int)((cur >> 16) & 0xFFu), x = (int)(cur >> 24);
         if (rows_ok) r = rows;
-        if (wpb == 2 || wpb == 4) w = wpb;
+        if (wpb == 4) w = wpb;
         if (xcd_remap >= 0 && xcd_remap <= 3) x = xcd_remap;
         const uint32_t next = (uint32_t)r | ((uint32_t)w << 16) | ((uint32_t)x << 24);
         if (g_shape.compare_exchange_weak(cur, next)) return;
@@ -77,7 +78,8 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     q.bl_on = f.bl_on;
     for (int k = 0; k < 4; ++k) q.bl_i[k] = f.bl_on ? (int)f.bl_q[k] : 0;
     q.th = th == 2 ? 2 : 4;
-    q.wpb = wpb == 2 ? 2 : 4;
+    q.wpb = 4;
+    (void)wpb;
     q.xcd_remap = xcd;
     q.aligned = 1;
     q.nsx = strip_count(f.w, f.out16 != 0);

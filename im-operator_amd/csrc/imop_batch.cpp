@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -22,17 +23,21 @@
 
 namespace imop_hip {
 
-hipError_t launch_batch_pattern0(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
-hipError_t launch_batch_pattern1(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
-hipError_t launch_batch_pattern2(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
-hipError_t launch_batch_pattern3(const BatchArgs& a, int alg, int rows, int blocks, hipStream_t st);
+#define IMOP_BATCH_PATTERN_DECL(k)                                                                             \
+    hipError_t launch_batch_pattern##k(const BatchArgs& a, int alg, int fmt, bool odd, int rows, bool persist, \
+                                       int blocks, hipStream_t st);
+IMOP_BATCH_PATTERN_DECL(0)
+IMOP_BATCH_PATTERN_DECL(1)
+IMOP_BATCH_PATTERN_DECL(2)
+IMOP_BATCH_PATTERN_DECL(3)
 
-hipError_t launch_batch(const BatchArgs& a, int pattern, int alg, int rows, int blocks, hipStream_t st) {
+hipError_t launch_batch(const BatchArgs& a, int pattern, int alg, int fmt, bool odd, int rows, bool persist,
+                        int blocks, hipStream_t st) {
     switch (pattern) {
-    case 0: return launch_batch_pattern0(a, alg, rows, blocks, st);
-    case 1: return launch_batch_pattern1(a, alg, rows, blocks, st);
-    case 2: return launch_batch_pattern2(a, alg, rows, blocks, st);
-    default: return launch_batch_pattern3(a, alg, rows, blocks, st);
+    case 0: return launch_batch_pattern0(a, alg, fmt, odd, rows, persist, blocks, st);
+    case 1: return launch_batch_pattern1(a, alg, fmt, odd, rows, persist, blocks, st);
+    case 2: return launch_batch_pattern2(a, alg, fmt, odd, rows, persist, blocks, st);
+    default: return launch_batch_pattern3(a, alg, fmt, odd, rows, persist, blocks, st);
     }
 }
 
@@ -59,8 +64,10 @@ int check_frame(const imop_hip_frame_desc& d) {
 
 namespace {
 
-// Launch shape of k_batch (imop_hip_set_batch_shape).
-std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
+// Launch shape of k_batch (imop_hip_set_batch_shape): rows per wave and
+// waves per workgroup of the group items, workgroups per CU of a persistent
+// grid (0: one workgroup per item), order (bit 1: strip-fastest).
+std::atomic<int> g_rows{2}, g_wpb{4}, g_bpc{0}, g_order{2};
 
 int cu_count() {
     static std::mutex mu;
@@ -77,10 +84,23 @@ int cu_count() {
     return cache[dev];
 }
 
+// One launch: the frames of one (pattern, algorithm, raw layout, width class).
 struct Group {
-    int pattern = 0, alg = 0;
+    int pattern = 0, alg = 0, fmt = 0;
+    bool odd = false;
     std::vector<BatchFrame> frames;
     uint32_t total = 0;
+    std::vector<uint16_t> chunk_frame;  // frame of item c * kBatchChunk
+
+    // The device copy: the frame table, then the chunk table at a 16-byte boundary.
+    size_t chunk_offset() const { return (frames.size() * sizeof(BatchFrame) + 15u) & ~(size_t)15u; }
+    size_t device_bytes() const { return chunk_offset() + chunk_frame.size() * sizeof(uint16_t); }
+    std::vector<uint8_t> device_image() const {
+        std::vector<uint8_t> b(device_bytes(), 0);
+        std::memcpy(b.data(), frames.data(), frames.size() * sizeof(BatchFrame));
+        std::memcpy(b.data() + chunk_offset(), chunk_frame.data(), chunk_frame.size() * sizeof(uint16_t));
+        return b;
+    }
 };
 
 struct Single {
@@ -92,7 +112,7 @@ struct Prepared {
     std::vector<Group> groups;
     std::vector<Single> singles;
     size_t prism_ws = 0;
-    int rows = 4, wpb = 4, bpc = 0, order = 2;
+    int rows = 2, wpb = 4, bpc = 0, order = 2;
 };
 
 int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
@@ -110,14 +130,16 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
     P.wpb = g_wpb;
     P.bpc = g_bpc;
     P.order = g_order;
-    std::map<std::pair<int, int>, size_t> index;
+    std::map<int, size_t> index;  // group of each launch key
     for (int i = 0; i < n; ++i) {
         const imop_hip_frame_desc& d = fr[i];
         int mode = -1;
         FrameParams f = make_params(d.in, d.in_bytes, d.out, d.width, d.height, d.pattern, d.bit_depth, d.is_packed,
                                     d.black_level, d.black_level_mode, d.algorithm, &mode);
         f.out16 = out_elem_of(d) == 2;
-        const int nsx = strip_count(d.width, false), nby = (d.height + P.rows - 1) / P.rows;
+        // the single-frame kernel's items: aligned strips for u16 RGB, group rows of rows x wpb
+        const int nsx = strip_count(d.width, f.out16 != 0);
+        const int nby = (d.height + P.rows * P.wpb - 1) / (P.rows * P.wpb);
         // k_batch runs MG 8-bit frames on the integer path; a packed 8-bit
         // frame without black level takes the exact path, so it runs alone
         const bool mode_ok = d.algorithm == kHqli || (d.algorithm == kMg && (f.fmt == FMT_U8) == (mode == MODE_MG_INT8));
@@ -130,17 +152,21 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
             if (d.algorithm == kPrism || d.algorithm == kRcd) P.prism_ws = std::max(P.prism_ws, prism_workspace_bytes(d.width, d.height));
             continue;
         }
-        const auto key = std::make_pair(d.pattern, d.algorithm == kMg ? 1 : 0);
+        const int alg = d.algorithm == kMg ? 1 : 0, odd = (d.width & 3) == 2 ? 1 : 0;
+        const int key = ((d.pattern * 2 + alg) * 8 + f.fmt) * 2 + odd;
         auto it = index.find(key);
         if (it == index.end()) {
             it = index.emplace(key, P.groups.size()).first;
             P.groups.emplace_back();
-            P.groups.back().pattern = key.first;
-            P.groups.back().alg = key.second;
+            P.groups.back().pattern = d.pattern;
+            P.groups.back().alg = alg;
+            P.groups.back().fmt = f.fmt;
+            P.groups.back().odd = odd != 0;
         }
         Group& g = P.groups[it->second];
         const uint64_t items = (uint64_t)nsx * (uint64_t)nby;
-        if ((uint64_t)g.total + items >= (1ull << 31)) return kInternal;
+        // 31-bit item numbers; frame numbers fit the 16-bit chunk table
+        if ((uint64_t)g.total + items >= (1ull << 31) || g.frames.size() >= 65535u) return kInternal;
         BatchFrame b{};
         b.in = f.in;
         b.out = f.out;
@@ -159,10 +185,21 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         g.frames.push_back(b);
         g.total += (uint32_t)items;
     }
+    for (Group& g : P.groups) {
+        const uint32_t nchunks = (g.total + kBatchChunk - 1) / kBatchChunk;
+        g.chunk_frame.resize(nchunks);
+        size_t fi = 0;
+        for (uint32_t c = 0; c < nchunks; ++c) {
+            const uint32_t item = c * (uint32_t)kBatchChunk;
+            while (fi + 1 < g.frames.size() && g.frames[fi + 1].first <= item) ++fi;
+            g.chunk_frame[c] = (uint16_t)fi;
+        }
+    }
     return kOk;
 }
 
-// Enqueue every launch of a prepared batch; tables[i] = device copy of groups[i].frames.
+// Enqueue every launch of a prepared batch; tables[i] = device copy of
+// groups[i] (Group::device_image).
 int enqueue(const Prepared& P, const std::vector<void*>& tables, void* prism_ws, hipStream_t st) {
     const int cus = cu_count();
     for (size_t i = 0; i < P.groups.size(); ++i) {
@@ -170,13 +207,15 @@ int enqueue(const Prepared& P, const std::vector<void*>& tables, void* prism_ws,
         if (g.total == 0) continue;
         BatchArgs a;
         a.frames = (const BatchFrame*)tables[i];
+        a.chunk_frame = (const uint16_t*)((const uint8_t*)tables[i] + g.chunk_offset());
         a.nframes = (int)g.frames.size();
         a.total = g.total;
         a.wpb = P.wpb;
         a.order = P.order;
-        int64_t blocks = ((int64_t)g.total + P.wpb - 1) / P.wpb;
+        int64_t blocks = (int64_t)g.total;  // one workgroup per item
         if (P.bpc > 0 && cus > 0) blocks = std::min<int64_t>(blocks, (int64_t)P.bpc * cus);
-        if (launch_batch(a, g.pattern, g.alg, P.rows, (int)blocks, st) != hipSuccess) return kInternal;
+        if (launch_batch(a, g.pattern, g.alg, g.fmt, g.odd, P.rows, P.bpc > 0, (int)blocks, st) != hipSuccess)
+            return kInternal;
     }
     for (const Single& s : P.singles) {
         const int e = launch_frame(s.f, s.algo, s.mode, prism_ws, st);
@@ -228,10 +267,11 @@ int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stre
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return kInternal;
     // work lists live in stream-ordered allocations released after the launch
     std::vector<void*> tables(P.groups.size(), nullptr);
+    std::vector<std::vector<uint8_t>> images(P.groups.size());
     for (size_t i = 0; i < P.groups.size(); ++i) {
-        const size_t bytes = P.groups[i].frames.size() * sizeof(BatchFrame);
-        if (hipMallocAsync(&tables[i], bytes, st) != hipSuccess ||
-            hipMemcpyAsync(tables[i], P.groups[i].frames.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+        images[i] = P.groups[i].device_image();
+        if (hipMallocAsync(&tables[i], images[i].size(), st) != hipSuccess ||
+            hipMemcpyAsync(tables[i], images[i].data(), images[i].size(), hipMemcpyHostToDevice, st) != hipSuccess) {
             e = kInternal;
             break;
         }
@@ -258,9 +298,9 @@ int imop_hip_batch_plan_create(const imop_hip_frame_desc* frames, int n, imop_hi
     }
     p->tables.assign(P.groups.size(), nullptr);
     for (size_t i = 0; i < P.groups.size(); ++i) {
-        const size_t bytes = P.groups[i].frames.size() * sizeof(BatchFrame);
-        if (hipMalloc(&p->tables[i], bytes) != hipSuccess ||
-            hipMemcpy(p->tables[i], P.groups[i].frames.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        const std::vector<uint8_t> image = P.groups[i].device_image();
+        if (hipMalloc(&p->tables[i], image.size()) != hipSuccess ||
+            hipMemcpy(p->tables[i], image.data(), image.size(), hipMemcpyHostToDevice) != hipSuccess) {
             free_plan(p);
             return kInternal;
         }
@@ -305,8 +345,8 @@ int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan) {
 }
 
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order) {
-    if (rows_per_wave == 4 || rows_per_wave == 8) g_rows = rows_per_wave;
-    if (waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4) g_wpb = waves_per_block;
+    if (rows_per_wave == 2 || rows_per_wave == 4) g_rows = rows_per_wave;
+    if (waves_per_block == 4) g_wpb = waves_per_block;
     if (blocks_per_cu >= 0 && blocks_per_cu <= 64) g_bpc = blocks_per_cu;
     if (order >= 0 && order <= 3) g_order = order & 2;
     return kOk;

@@ -130,15 +130,53 @@ DefectState* defect_state(int dev) {
     return g_defect[dev].get();
 }
 
+// Scratch of one call: the input copy, 256 bytes of pass control words and
+// two flag bytes per tile.
+struct DefectScratch {
+    size_t frame, frame_al, need;
+    DefectScratch(int w, int h, int bd) {
+        frame = (size_t)w * (size_t)h * (bd > 8 ? 2u : 1u);
+        frame_al = (frame + 255u) & ~(size_t)255u;
+        need = frame_al + 256u + 2u * (size_t)defect_tiles_x(w) * (size_t)defect_tiles_y(h);
+    }
+};
+
+// The replacement threshold: (int)(std::max(0.05f, std::min(1.0f, threshold))
+// * max) (adaptive.cpp:17-18; NaN -> 1).
+int defect_threshold(float threshold, int bd) {
+    const float t_lo = (threshold < 1.0f) ? threshold : 1.0f;
+    const float t = (0.05f < t_lo) ? t_lo : 0.05f;
+    const int mv = bd <= 0 ? 255 : bd > 16 ? 65535 : (int)((1u << bd) - 1u);  // detail::safe_max_val
+    return (int)(t * (float)mv);
+}
+
+// Stream-ordered form: the input copy, then `passes` passes (each returns at
+// once after a pass that changed nothing), then the status word; the scratch
+// is this call's own stream-ordered allocation (graph-owned under capture).
+int defect_adaptive_async(uint8_t* data, int w, int h, int bd, float threshold, int passes, int* d_status,
+                          hipStream_t st) {
+    const DefectScratch sc(w, h, bd);
+    void* d = nullptr;
+    if (planes_alloc(sc.need, st, &d) != hipSuccess) return kInternal;
+    uint8_t* orig = (uint8_t*)d;
+    int* ctl = (int*)(orig + sc.frame_al);
+    uint8_t* flags = orig + sc.frame_al + 256u;
+    const int thresh = defect_threshold(threshold, bd);
+    hipError_t e = launch_defect_copy(data, orig, sc.frame, ctl, st);
+    for (int pass = 0; e == hipSuccess && pass < passes; ++pass)
+        e = launch_defect_pass(data, orig, flags, ctl, w, h, bd, thresh, pass, st);
+    if (e == hipSuccess) e = launch_defect_status(ctl, passes, d_status, st);
+    const hipError_t fe = hipFreeAsync(d, st);
+    return e == hipSuccess && fe == hipSuccess ? kOk : kInternal;
+}
+
 int defect_adaptive(uint8_t* data, int w, int h, int bd, float threshold, hipStream_t st) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return kInternal;
     DefectState* S = defect_state(dev);
     std::lock_guard<std::mutex> lk(S->mu);
-    const size_t frame = (size_t)w * (size_t)h * (bd > 8 ? 2u : 1u);
-    const size_t frame_al = (frame + 255u) & ~(size_t)255u;
-    const int ntiles = defect_tiles_x(w) * defect_tiles_y(h);
-    const size_t need = frame_al + 256u + 2u * (size_t)ntiles;
+    const DefectScratch sc(w, h, bd);
+    const size_t frame = sc.frame, frame_al = sc.frame_al, need = sc.need;
     if (S->bytes < need) {
         if (S->d) {
             if (hipDeviceSynchronize() != hipSuccess) return kInternal;
@@ -156,11 +194,7 @@ int defect_adaptive(uint8_t* data, int w, int h, int bd, float threshold, hipStr
     uint8_t* orig = (uint8_t*)S->d;
     int* ctl = (int*)(orig + frame_al);
     uint8_t* flags = orig + frame_al + 256u;
-    // std::max(0.05f, std::min(1.0f, threshold)) (adaptive.cpp:17-18; NaN -> 1)
-    const float t_lo = (threshold < 1.0f) ? threshold : 1.0f;
-    const float t = (0.05f < t_lo) ? t_lo : 0.05f;
-    const int mv = bd <= 0 ? 255 : bd > 16 ? 65535 : (int)((1u << bd) - 1u);  // detail::safe_max_val
-    const int thresh = (int)(t * (float)mv);
+    const int thresh = defect_threshold(threshold, bd);
     if (launch_defect_copy(data, orig, frame, ctl, st) != hipSuccess) return kInternal;
     // A pass fixes at least the next pixel of every chain of dependencies, and
     // the longest chain on a colour plane of pw x ph pixels is pw + 2 ph long.
@@ -181,6 +215,18 @@ int defect_adaptive(uint8_t* data, int w, int h, int bd, float threshold, hipStr
         if (pass >= cap) return kInternal;
         chunk = std::min(chunk * 2, 64);
     }
+}
+
+// defect_correct::validate_defect_correct_inputs (src/defect_correct/dispatch.cpp:7-15)
+// and process_adaptive's size check (adaptive.cpp:13), then the buffer.
+int defect_check(const void* d_data, uint64_t bytes, int width, int height, int bit_depth) {
+    if (!d_data) return kNull;
+    if (width <= 0 || height <= 0) return kDims;
+    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
+    if (width < 5 || height < 5) return kSmall;
+    const uint64_t need = (uint64_t)width * (uint64_t)height * (bit_depth > 8 ? 2u : 1u);
+    if (bytes < need || (bit_depth > 8 && ((uintptr_t)d_data & 1))) return kInternal;
+    return kOk;
 }
 
 }  // namespace
@@ -288,15 +334,21 @@ int imop_hip_defect_adaptive(void* d_data, uint64_t bytes, int width, int height
                              float threshold, void* stream) {
     (void)pattern;  // same-colour neighbours at distance 2 whatever the CFA order
     if (device_count_cached() == 0) return kInternal;
-    // defect_correct::validate_defect_correct_inputs (src/defect_correct/dispatch.cpp:7-15)
-    if (!d_data) return kNull;
-    if (width <= 0 || height <= 0) return kDims;
-    if (bit_depth < 1 || bit_depth > 16) return kBitDepth;
-    // process_adaptive (adaptive.cpp:13)
-    if (width < 5 || height < 5) return kSmall;
-    const uint64_t need = (uint64_t)width * (uint64_t)height * (bit_depth > 8 ? 2u : 1u);
-    if (bytes < need || (bit_depth > 8 && ((uintptr_t)d_data & 1))) return kInternal;
+    const int e = defect_check(d_data, bytes, width, height, bit_depth);
+    if (e != kOk) return e;
     return defect_adaptive((uint8_t*)d_data, width, height, bit_depth, threshold, (hipStream_t)stream);
+}
+
+int imop_hip_defect_adaptive_async(void* d_data, uint64_t bytes, int width, int height, int pattern,
+                                   int bit_depth, float threshold, int max_passes, int* d_status, void* stream) {
+    (void)pattern;
+    if (device_count_cached() == 0) return kInternal;
+    const int e = defect_check(d_data, bytes, width, height, bit_depth);
+    if (e != kOk) return e;
+    if (!d_status) return kNull;
+    if (max_passes < 1 || max_passes > 65536) return kInternal;
+    return defect_adaptive_async((uint8_t*)d_data, width, height, bit_depth, threshold, max_passes, d_status,
+                                 (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -225,6 +225,8 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_lens_shading_poly.restype = i
         L.imop_hip_defect_adaptive.argtypes = [vp, u64, i, i, i, i, ctypes.c_float, vp]
         L.imop_hip_defect_adaptive.restype = i
+        L.imop_hip_defect_adaptive_async.argtypes = [vp, u64, i, i, i, i, ctypes.c_float, i, vp, vp]
+        L.imop_hip_defect_adaptive_async.restype = i
         L.imop_hip_analyze_data.argtypes = [vp, u64, ctypes.POINTER(DataInfo), vp]
         L.imop_hip_analyze_data.restype = i
         L.imop_hip_guess_pattern.argtypes = [vp, u64, i, i, i, i, ctypes.POINTER(i), vp]
@@ -371,6 +373,19 @@ def defect_correct(d_bayer, width: int, height: int, pattern: int, bit_depth: in
     return lib().imop_hip_defect_adaptive(ctypes.c_void_p(d_bayer.data_ptr()),
                                           d_bayer.numel() * d_bayer.element_size(), width, height, int(pattern),
                                           bit_depth, ctypes.c_float(threshold), ctypes.c_void_p(_stream_arg(stream)))
+
+
+def defect_correct_async(d_bayer, width: int, height: int, pattern: int, bit_depth: int, d_status,
+                         threshold: float = 0.3, max_passes: int = 32, stream=None) -> int:
+    """imop_hip_defect_adaptive_async: the same correction stream-ordered
+    (capturable into a graph); writes 0 / 1 / 2 (converged / not within
+    max_passes / error) to d_status, a one-element int32 device tensor, when
+    the passes end."""
+    return lib().imop_hip_defect_adaptive_async(ctypes.c_void_p(d_bayer.data_ptr()),
+                                                d_bayer.numel() * d_bayer.element_size(), width, height,
+                                                int(pattern), bit_depth, ctypes.c_float(threshold), int(max_passes),
+                                                ctypes.c_void_p(d_status.data_ptr()),
+                                                ctypes.c_void_p(_stream_arg(stream)))
 
 
 def analyze_data(d_data, stream=None) -> tuple[int, dict]:

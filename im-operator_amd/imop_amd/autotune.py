@@ -1,9 +1,10 @@
 """Autotuner for the launch shape of the register-resident HQLI / MG kernel.
 
 The shape is (rows each wave computes, waves per group, XCD remap): a group
-of waves stacked down one strip shares its decoded rows and colour
-differences through LDS, so rows x waves sets the vertical apron overhead
-(6 extra decoded rows per group) against the number of groups in flight;
+of 4 waves stacked down one strip shares its decoded rows and colour
+differences through LDS, so the rows set the vertical apron overhead (6
+extra decoded rows per group) against the output rows a group keeps in
+flight;
 the XCD remap and group order set how groups are spread over the 8 XCDs and
 their L2s.  Each candidate is timed on
 device-resident synthetic frames of the requested class with HIP events,
@@ -26,7 +27,7 @@ import numpy as np
 import imop_amd
 from imop_amd import scene
 
-CANDIDATES = list(itertools.product((2, 4), (2, 4), (0, 1, 2, 3)))
+CANDIDATES = list(itertools.product((2, 4), (4,), (0, 1, 2, 3)))
 
 
 def frame_class(w: int, h: int, bd: int, packed: bool, algo: int) -> str:
@@ -85,7 +86,7 @@ def tune(w: int, h: int, bd: int, packed: bool, algo: int, black_level=None, see
     return best[0], best[1], best[2], best[3], table
 
 
-BATCH_CANDIDATES = list(itertools.product((4, 8), (1, 2, 4), (0, 2, 4, 8), (0, 2)))
+BATCH_CANDIDATES = list(itertools.product((2, 4), (4,), (0, 2, 4, 8), (0, 2)))
 
 
 def tune_batch(desc_sets, digest=None, candidates=BATCH_CANDIDATES, iters: int = 8, verbose: bool = True):

@@ -105,7 +105,7 @@ int imop_hip_black_level_host(uint8_t* data, int width, int height, int pattern,
 
 /* Launch shape of the register-resident HQLI / MG kernel, chosen by the
  * autotuner (im-operator_amd/imop_amd/autotune.py): rows each wave
- * computes (2 or 4), waves per workgroup stacked down one strip (2 or 4),
+ * computes (2 or 4), waves per workgroup stacked down one strip (4),
  * and the workgroup order: bit 0 remaps workgroups so each XCD works on a
  * contiguous run of them (else launch order), bit 1 orders them
  * strip-fastest (adjacent workgroups cover adjacent columns of the same
@@ -183,11 +183,13 @@ int imop_hip_batch_plan_create(const imop_hip_frame_desc* frames, int n, imop_hi
 int imop_hip_batch_plan_launch(imop_hip_batch_plan* plan, void* stream);
 int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan);
 
-/* Launch shape of the batched kernel, chosen by the autotuner: rows per band
- * (4 or 8), waves per workgroup (1, 2 or 4), workgroups per CU of the
- * persistent grid (1..64; 0 = one wave per work item), and wave order
- * (bit 1: strip-fastest within a frame).  Out-of-range values leave a setting
- * unchanged.  Process-wide; no counterpart in the reference. */
+/* Launch shape of the batched kernel, chosen by the autotuner.  Its work
+ * items are the single-frame kernel's workgroups: rows each wave computes
+ * (2 or 4) times waves per workgroup stacked down one strip (4);
+ * workgroups per CU of the persistent grid (1..64; 0 = one workgroup per
+ * item); order (bit 1: strip-fastest within a frame).  Initial shape
+ * (2, 4, 0, 2).  Out-of-range values leave a setting unchanged.
+ * Process-wide; no counterpart in the reference. */
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order);
 int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order);
 
@@ -353,6 +355,24 @@ int imop_hip_demosaic_pipeline(const void* d_in, uint64_t in_bytes, void* d_out,
  * 6 InternalError). */
 int imop_hip_defect_adaptive(void* d_data, uint64_t bytes, int width, int height, int pattern, int bit_depth,
                              float threshold, void* stream);
+
+/* The same correction, stream-ordered: enqueues the input copy and
+ * max_passes fixed-point passes (1..65536) on `stream` and returns without
+ * waiting, so it can be captured into a HIP graph.  Each pass after one that
+ * changed nothing returns at once.  When the passes end it writes *d_status
+ * (one int of device memory): 0 -- the last pass changed nothing, so the frame
+ * equals imop_hip_defect_adaptive's result; 1 -- the frame still changed in
+ * the last pass (it holds an intermediate iterate; run the synchronous form
+ * on the original frame, or allow more passes); 2 -- internal error.  Natural
+ * frames settle in 2-4 passes; a 12 MP frame of pure noise at threshold 0.05
+ * needs 28.  Scratch (a copy of the frame, control words, two bytes per
+ * 128 x 32 tile) is this call's own stream-ordered allocation, graph-owned
+ * under capture, so calls on different streams never share it.  Checks and
+ * error codes as imop_hip_defect_adaptive, plus 1 for a NULL d_status and 6
+ * for max_passes out of range.  No counterpart in the reference (its
+ * correction is a synchronous CPU loop). */
+int imop_hip_defect_adaptive_async(void* d_data, uint64_t bytes, int width, int height, int pattern,
+                                   int bit_depth, float threshold, int max_passes, int* d_status, void* stream);
 
 /* --------------------------------------------------------- raw analyzer */
 

@@ -107,15 +107,17 @@ def test_launch_shape_auto_rows_round_trip():
         assert imop_amd.get_launch_shape() == (0, 4, 3)
         imop_amd.set_launch_shape(*imop_amd.get_launch_shape())
         assert imop_amd.get_launch_shape() == (0, 4, 3)
-        imop_amd.set_launch_shape(2, 2, 1)
-        assert imop_amd.get_launch_shape() == (2, 2, 1)
+        imop_amd.set_launch_shape(2, 4, 1)
+        assert imop_amd.get_launch_shape() == (2, 4, 1)
         imop_amd.set_launch_shape(-1, 0, -1)
-        assert imop_amd.get_launch_shape() == (2, 2, 1)
-        # rows 2 / 4 and groups of 2 / 4 waves only
-        imop_amd.set_launch_shape(8, 1, 7)
-        assert imop_amd.get_launch_shape() == (2, 2, 1)
+        assert imop_amd.get_launch_shape() == (2, 4, 1)
+        # rows 2 / 4 and groups of 4 waves only
+        imop_amd.set_launch_shape(8, 2, 7)
+        assert imop_amd.get_launch_shape() == (2, 4, 1)
+        imop_amd.set_launch_shape(4, 1, 2)
+        assert imop_amd.get_launch_shape() == (4, 4, 2)
         imop_amd.set_launch_shape(0)
-        assert imop_amd.get_launch_shape() == (0, 2, 1)
+        assert imop_amd.get_launch_shape() == (0, 4, 2)
     finally:
         imop_amd.set_launch_shape(*saved)
 

@@ -104,8 +104,8 @@ def test_ragged_batch_plan_replays_match_oracle():
             _check_against_oracle(frames)
 
 
-@pytest.mark.parametrize("rows", [4, 8])
-@pytest.mark.parametrize("wpb", [1, 2, 4])
+@pytest.mark.parametrize("rows", [2, 4])
+@pytest.mark.parametrize("wpb", [4])
 @pytest.mark.parametrize("bpc", [0, 1, 3, 16])
 @pytest.mark.parametrize("order", [0, 2])
 def test_every_batch_shape(rows, wpb, bpc, order, restore_batch_shape):
@@ -120,8 +120,9 @@ def test_every_batch_shape(rows, wpb, bpc, order, restore_batch_shape):
 
 
 def test_tiny_grid_covers_every_item(restore_batch_shape):
-    """Fewer workgroups than XCDs: every work item still runs."""
-    imop_amd.set_batch_shape(4, 1, 1, 2)
+    """Fewer workgroups than XCDs: every work item still runs (a 256 x 16
+    frame is 2 strips x one 16-row group row: 2 workgroups)."""
+    imop_amd.set_batch_shape(4, 4, 1, 2)
     frames = _materialise([dict(w=256, h=16, bd=12, packed=True, algo=MG, pattern=0, bl=None, out_elem=0, seed=5)])
     e, failed = imop_amd.demosaic_batch(_descs(frames))
     torch.cuda.synchronize()

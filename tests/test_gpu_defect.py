@@ -132,3 +132,115 @@ def test_invalid_arguments():
     assert imop_amd.defect_correct(d, 64, 4, 0, 12) == 5
     assert imop_amd.defect_correct(d, 64, 65, 0, 12) == 6                                        # buffer too small
     assert not d.any()
+
+
+# ---------------------------------------------------------------- stream-ordered form
+
+def _run_async(img, bd, thr, max_passes, stream=None):
+    h, w = img.shape
+    d = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(DEV)
+    status = torch.full((1,), -1, dtype=torch.int32, device=DEV)
+    assert imop_amd.defect_correct_async(d, w, h, 0, bd, status, thr, max_passes, stream=stream) == 0
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    return (out.view(np.uint16) if bd > 8 else out).reshape(h, w), int(status.item())
+
+
+@pytest.mark.parametrize("bd", [8, 12, 16])
+@pytest.mark.parametrize("kind,thr", [("noise", 0.05), ("sparse", 0.3), ("clusters", 0.1), ("flat", 0.3)])
+def test_async_matches_oracle(bd, kind, thr):
+    """Enough passes: status 0 and the oracle's frame, byte for byte."""
+    h, w = 70, 301
+    img = _img(kind, bd, h, w, bd * 7 + len(kind))
+    print(f"defect async bd {bd} {kind} thr {thr}", flush=True)
+    got, status = _run_async(img, bd, thr, 256)
+    assert status == 0
+    np.testing.assert_array_equal(got, O.defect_adaptive(img, bd, thr)[1])
+
+
+def test_async_reports_unconverged_frames():
+    """Pure noise at the lowest threshold needs several passes: one pass
+    reports 1; the passes the synchronous form takes report 0."""
+    img = _img("noise", 12, 96, 256, 3)
+    want = O.defect_adaptive(img, 12, 0.05)[1]
+    got, status = _run_async(img, 12, 0.05, 1)
+    assert status == 1
+    assert not np.array_equal(got, want)
+    got, status = _run_async(img, 12, 0.05, 512)
+    assert status == 0
+    np.testing.assert_array_equal(got, want)
+
+
+def test_async_full_frame_noise():
+    """12 MP 12-bit pure noise at threshold 0.05 within 64 passes."""
+    img = _img("noise", 12, 3032, 4024, 17)
+    print("defect async full frame noise", flush=True)
+    got, status = _run_async(img, 12, 0.05, 64)
+    assert status == 0
+    np.testing.assert_array_equal(got, O.defect_adaptive(img, 12, 0.05)[1])
+
+
+def test_async_two_streams_no_sync():
+    """Two frames corrected on two streams with no synchronisation between the
+    calls: each call's scratch is its own."""
+    imgs = [_img("clusters", 12, 300, 500, 1), _img("noise", 12, 320, 480, 2)]
+    thrs = [0.1, 0.05]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    ds, sts = [], []
+    for img, thr, s in zip(imgs, thrs, streams):
+        h, w = img.shape
+        d = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(DEV)
+        st = torch.full((1,), -1, dtype=torch.int32, device=DEV)
+        torch.cuda.synchronize()
+        assert imop_amd.defect_correct_async(d, w, h, 0, 12, st, thr, 128, stream=s) == 0
+        ds.append(d)
+        sts.append(st)
+    torch.cuda.synchronize()
+    for img, thr, d, st in zip(imgs, thrs, ds, sts):
+        assert int(st.item()) == 0
+        got = d.cpu().numpy().view(np.uint16).reshape(img.shape)
+        np.testing.assert_array_equal(got, O.defect_adaptive(img, 12, thr)[1])
+
+
+def test_async_graph_capture_replays():
+    """Input copy + stream-ordered correction captured into a graph, replayed
+    three times with eager work between the replays; every replay equals the
+    oracle and reports 0."""
+    img = _img("clusters", 12, 200, 333, 9)
+    h, w = img.shape
+    want = O.defect_adaptive(img, 12, 0.1)[1]
+    src = torch.from_numpy(img.view(np.uint8).ravel().copy()).to(DEV)
+    d = torch.empty_like(src)
+    st = torch.zeros((1,), dtype=torch.int32, device=DEV)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        d.copy_(src)
+        assert imop_amd.defect_correct_async(d, w, h, 0, 12, st, 0.1, 64) == 0
+    for rep in range(3):
+        print(f"defect graph replay {rep}", flush=True)
+        d.fill_(0x33)
+        st.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert int(st.item()) == 0
+        np.testing.assert_array_equal(d.cpu().numpy().view(np.uint16).reshape(h, w), want)
+
+
+def test_async_invalid_arguments():
+    d = torch.zeros(64 * 64 * 2, dtype=torch.uint8, device=DEV)
+    st = torch.zeros((1,), dtype=torch.int32, device=DEV)
+    L = imop_amd.lib()
+    f = ctypes.c_float(0.3)
+    assert L.imop_hip_defect_adaptive_async(None, 0, 64, 64, 0, 12, f, 8, ctypes.c_void_p(st.data_ptr()), None) == 1
+    assert imop_amd.defect_correct_async(d, 0, 64, 0, 12, st) == 2
+    assert imop_amd.defect_correct_async(d, 64, 64, 0, 17, st) == 3
+    assert imop_amd.defect_correct_async(d, 4, 64, 0, 12, st) == 5
+    assert imop_amd.defect_correct_async(d, 64, 65, 0, 12, st) == 6
+    assert L.imop_hip_defect_adaptive_async(ctypes.c_void_p(d.data_ptr()), d.numel(), 64, 64, 0, 12, f, 8, None,
+                                            None) == 1
+    assert imop_amd.defect_correct_async(d, 64, 64, 0, 12, st, max_passes=0) == 6
+    assert imop_amd.defect_correct_async(d, 64, 64, 0, 12, st, max_passes=65537) == 6
+    torch.cuda.synchronize()
+    assert not d.any() and not st.any()
