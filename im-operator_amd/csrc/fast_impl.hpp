@@ -972,8 +972,10 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
     else if (MODE != MODE_HQLI) stage = Lw + pw * 128;  // own decoded rows, read above
     else stage = Lw + G::kDOff / 4 + W * 384;            // HQLI: no colour-difference rows
     // MG into u16 RGB without the epilogue: interpolation and whole-chunk
-    // staged stores inline; every other form through band_step
-    constexpr bool lean = MODE != MODE_HQLI && FMT != FMT_U8 && !EPI && !HC;
+    // staged stores inline; every other form through band_step (8-bit
+    // frames write u16 RGB only on request, as batch frames do)
+    constexpr bool lean_form = MODE != MODE_HQLI && !EPI && !HC;
+    const bool lean = lean_form && (FMT != FMT_U8 || f.out16);
     const int ch = MODE == MODE_MG_INT8 ? 255 : f.clamp_hi;
 #pragma unroll
     for (int s = 0; s < TH / 2; ++s) {
@@ -995,7 +997,7 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
                 d_from_own<PAT>((t + 1) & 1, (int)Lw[G::kDOff / 4 + (jb + t) * 128 + lane],
                                 (int)Lw[G::kDOff / 4 + (jb + t) * 128 + 64 + lane], Dw[t]);
         }
-        if (lean) {
+        if (lean_form && lean) {
             int R[2][4], Gc[2][4], B[2][4];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
