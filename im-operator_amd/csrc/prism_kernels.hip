@@ -209,26 +209,74 @@ struct Tile {
     static constexpr int W = TW + 2 * A, H = TH + 2 * A;
 };
 
+// (float)v / m rounded to nearest for an integer sample v in [0, 65535] and
+// m = 2^bd - 1, bd in 1..16, from y = 1 / m rounded to nearest: the product
+// and one fused remainder correction.  Checked against the correctly rounded
+// quotient for every such v and m on the CPU (tests/cfa_norm.c).
+__device__ __forceinline__ float norm_sample(int v, float m, float y) {
+    const float a = (float)v;
+    const float q0 = a * y;
+    return fmaf(fmaf(-m, q0, a), y, q0);
+}
+
+// LDS copy (row pitch RW) of the RW x RH window of a frame-sized plane whose
+// top-left frame position is (xs, ys); 0 outside the frame.  Each wave takes
+// two rows per step (lanes 0-31 the first, 32-63 the second), each lane two
+// consecutive columns: no index division, and the bounds checks only in
+// windows that cross the frame edge.  RW even, at most 64; 256 threads;
+// frames of fewer than 2^31 pixels (32-bit offsets).
+template <int RW, int RH>
+__device__ __forceinline__ void fill_plane(float* dst, const float* __restrict__ src, int xs, int ys, int w,
+                                           int h) {
+    static_assert(RW % 2 == 0 && RW <= 64, "fill_plane: RW");
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = 2 * (lane & 31);
+    if (c >= RW) return;
+    const bool interior = xs >= 0 && ys >= 0 && xs + RW <= w && ys + RH <= h;  // workgroup-uniform
+    for (int r = 2 * wv + (lane >> 5); r < RH; r += 8) {
+        const int x = xs + c, y = ys + r;
+        float2 v;
+        if (interior) {
+            const float* p = src + (uint32_t)(y * w + x);
+            v = make_float2(p[0], p[1]);
+        } else {
+            const bool yok = (unsigned)y < (unsigned)h;
+            v.x = yok && (unsigned)x < (unsigned)w ? src[(uint32_t)(y * w + x)] : 0.0f;
+            v.y = yok && (unsigned)(x + 1) < (unsigned)w ? src[(uint32_t)(y * w + x + 1)] : 0.0f;
+        }
+        *(float2*)(dst + r * RW + c) = v;
+    }
+}
+
 // ------------------------------------------------------------------ pass A
 
 constexpr int AC = 5;  // CFA apron of pass A
 
 template <bool RCD>
 __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
-    __shared__ float s_cfa[Tile<AC>::H][Tile<AC>::W];  // tile + 5
-    __shared__ float s_vh[Tile<1>::H][Tile<1>::W];     // tile + 1
-    __shared__ float s_lp[Tile<2>::H][Tile<2>::W];     // tile + 2
+    __shared__ __align__(16) float s_cfa[Tile<AC>::H][Tile<AC>::W];  // tile + 5
+    __shared__ __align__(16) float s_vh[Tile<1>::H][Tile<1>::W];     // tile + 1
+    __shared__ __align__(16) float s_lp[Tile<2>::H][Tile<2>::W];     // tile + 2
     const int w = f.w, h = f.h;
     int x0, y0;
     tile_origin(w, h, x0, y0);
     const PrismPlanes P = planes_of(ws, w, h);
-    const float scale = (float)f.maxv;
+    const float scale = (float)f.maxv, rscale = 1.0f / scale;
 
-    // S0: normalised CFA (prism.cpp:58-78)
-    for (int k = threadIdx.x; k < Tile<AC>::H * Tile<AC>::W; k += blockDim.x) {
-        const int j = k / Tile<AC>::W, i = k - j * Tile<AC>::W;
-        const int x = x0 - AC + i, y = y0 - AC + j;
-        s_cfa[j][i] = (x >= 0 && x < w && y >= 0 && y < h) ? (float)sample_at(f, x, y) / scale : 0.0f;
+    // S0: normalised CFA (prism.cpp:58-78); each wave takes whole rows
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        constexpr int RW = Tile<AC>::W;
+        static_assert(RW <= 64, "S0: one row per wave step");
+        if (lane < RW) {
+            const int x = x0 - AC + lane;
+            const bool xok = (unsigned)x < (unsigned)w;
+            for (int j = wv; j < Tile<AC>::H; j += 4) {
+                const int y = y0 - AC + j;
+                s_cfa[j][lane] = xok && (unsigned)y < (unsigned)h ? norm_sample(sample_at(f, x, y), scale, rscale)
+                                                                  : 0.0f;
+            }
+        }
     }
     __syncthreads();
     // S1 over tile + 1, S2 over tile + 2
@@ -286,30 +334,22 @@ __global__ __launch_bounds__(256) void k_prism_a(FrameParams f, void* ws) {
 // cover the tile + 4
 template <bool RCD>
 __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
-    __shared__ float s_cfa[Tile<7>::H][Tile<7>::W];
-    __shared__ float s_g[Tile<6>::H][Tile<6>::W];
-    __shared__ float s_pq[Tile<5>::H][Tile<5>::W];   // later: R after the median (tile)
-    __shared__ float s_vh[Tile<2>::H][Tile<2>::W];   // later: B after the median (tile)
-    __shared__ float s_r[Tile<4>::H][Tile<4>::W];    // chroma planes over tile + 4
-    __shared__ float s_b[Tile<4>::H][Tile<4>::W];
+    __shared__ __align__(16) float s_cfa[Tile<7>::H][Tile<7>::W];
+    __shared__ __align__(16) float s_g[Tile<6>::H][Tile<6>::W];
+    __shared__ __align__(16) float s_pq[Tile<5>::H][Tile<5>::W];   // later: R after the median (tile)
+    __shared__ __align__(16) float s_vh[Tile<2>::H][Tile<2>::W];   // later: B after the median (tile)
+    __shared__ __align__(16) float s_r[Tile<4>::H][Tile<4>::W];    // chroma planes over tile + 4
+    __shared__ __align__(16) float s_b[Tile<4>::H][Tile<4>::W];
     const int w = f.w, h = f.h;
     int x0, y0;
     tile_origin(w, h, x0, y0);
     const PrismPlanes P = planes_of(ws, w, h);
 
     // (frames on this path have fewer than 2^31 pixels: 32-bit offsets)
-    auto fill = [&](float* dst, int A, int RW, int RH, const float* src) {
-        for (int k = threadIdx.x; k < RW * RH; k += blockDim.x) {
-            const int j = k / RW, i = k - j * RW;
-            const int x = x0 - A + i, y = y0 - A + j;
-            const bool ok = (unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h;
-            dst[k] = ok ? src[(uint32_t)(y * w + x)] : 0.0f;
-        }
-    };
-    fill(&s_cfa[0][0], 7, Tile<7>::W, Tile<7>::H, P.cfa);
-    fill(&s_g[0][0], 6, Tile<6>::W, Tile<6>::H, P.c1);
-    fill(&s_pq[0][0], 5, Tile<5>::W, Tile<5>::H, P.pq);
-    fill(&s_vh[0][0], 2, Tile<2>::W, Tile<2>::H, P.vh);
+    fill_plane<Tile<7>::W, Tile<7>::H>(&s_cfa[0][0], P.cfa, x0 - 7, y0 - 7, w, h);
+    fill_plane<Tile<6>::W, Tile<6>::H>(&s_g[0][0], P.c1, x0 - 6, y0 - 6, w, h);
+    fill_plane<Tile<5>::W, Tile<5>::H>(&s_pq[0][0], P.pq, x0 - 5, y0 - 5, w, h);
+    fill_plane<Tile<2>::W, Tile<2>::H>(&s_vh[0][0], P.vh, x0 - 2, y0 - 2, w, h);
     __syncthreads();
 
     // chroma planes over tile + 4: at R/B sites the native and S5's opposite
