@@ -317,8 +317,17 @@ def bench_batch(steps, warmup, with_cpu, shape=None):
         graphs.append(g)
 
     def timed(fn):
+        # untimed steps for about a fifth of a second first, so the clocks
+        # have settled whichever form ran before
         for k in range(warmup):
             fn(k)
+        torch.cuda.synchronize()
+        t_warm, k = time.perf_counter(), 0
+        while time.perf_counter() - t_warm < 0.2 and k < 500:
+            fn(k)
+            k += 1
+            if k % 8 == 0:
+                torch.cuda.synchronize()
         torch.cuda.synchronize()
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)

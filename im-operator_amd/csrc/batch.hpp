@@ -22,27 +22,31 @@ struct BatchFrame {
     int32_t bl_i[4];       // integral black level by (y & 1) * 2 + (x & 1); 0 without
     uint16_t nsx, nby;     // strips (strip_count) and group rows (rows per wave x waves per workgroup)
     uint32_t first;        // index of the frame's first work item in the batch
+    uint64_t place_mul;    // place_reciprocal(nsx * nby, strip-fastest ? nsx : nby)
 };
-static_assert(sizeof(BatchFrame) == 72, "BatchFrame layout");
+static_assert(sizeof(BatchFrame) == 80, "BatchFrame layout");
 
-// Items per entry of the chunk table: chunk c's entry is the frame holding
-// item c * kBatchChunk, so an item's frame is that entry or a later one.
+// Items per entry of the chunk table.  Chunk c's entry {fi, end}: fi is the
+// frame holding item c * kBatchChunk and end the first item of frame fi + 1
+// (the batch's item count for the last frame), so an item of the chunk below
+// end belongs to fi and one at or past it to a later frame.
 constexpr int kBatchChunk = 32;
 
 // Kernel arguments of one batched launch.
 struct BatchArgs {
     const BatchFrame* frames;  // device table, sorted by `first`
-    const uint16_t* chunk_frame;  // device table: frame of item c * kBatchChunk, ceil(total / kBatchChunk) entries
+    const uint2* chunk;        // device table of ceil(total / kBatchChunk) entries {fi, end}
     int32_t nframes;
     uint32_t total;            // work items (frame, strip, group row) of all frames
-    int32_t wpb;               // waves per workgroup (stacked down the strip)
+    int32_t wpb;               // 4: group items (4 waves down a strip); 1: one wave per item
     int32_t order;             // bit 1: strip-fastest within a frame
 };
 
 // Enqueue k_batch for one layout class -- pattern, alg (0 HQLI, 1 MG), raw
 // layout fmt (FMT_U8 / U16 / P12 / P10), odd (w % 4 == 2) -- with `rows`
-// rows per wave (2 or 4), over `blocks` workgroups of 4 waves: one per item,
-// or a persistent grid walking the list.
+// rows per wave (group items of a.wpb = 4 waves: 2 or 4; wave items, a.wpb =
+// 1: 4 or 8), over `blocks` workgroups: one per item, or a persistent grid
+// walking the list.
 hipError_t launch_batch(const BatchArgs& a, int pattern, int alg, int fmt, bool odd, int rows, bool persist,
                         int blocks, hipStream_t st);
 

@@ -23,8 +23,14 @@
 // single-frame kernel places its groups.  PERSIST = true: a grid of a few
 // workgroups per CU walks each slice with a stride of that slice's workgroup
 // count (its loop keeps 13 more VGPRs live, 6 waves per SIMD).  Each item's
-// frame comes from a table giving the frame of every kBatchChunk-th item,
-// then a forward scan of the sorted frame table (scalar loads).
+// frame comes from a chunk table (item_frame) by scalar loads, and its strip
+// and group row from place_split with the frame's reciprocal, as in the
+// single-frame kernel: a batch of one 12 MP frame then runs in the
+// single-frame kernel's time.
+//
+// Wave form (k_batch_w, waves per item 1): every wave is one item, a strip
+// by a band of 4 or 8 rows on the single-frame kernel's per-wave path, in
+// workgroups of one wave placed like the group form's workgroups.
 #pragma once
 
 #include "batch.hpp"
@@ -32,19 +38,40 @@
 
 namespace imop_hip {
 
+// *slot, a pointer to global memory that is itself read from device memory,
+// loaded as a global pointer: addressing through it then compiles to global
+// loads and stores, as through the pointers read from the kernel arguments,
+// not to flat ones (which wait on both the LDS and the memory counters; the
+// 12 MP RAW12 batch item ran 1.2x slower with them).
+template <typename T>
+__device__ __forceinline__ T* load_global_ptr(T* const* slot) {
+    typedef __attribute__((address_space(1))) T gT;
+    return (T*)*(gT* const*)(const void*)slot;
+}
+
+// The frame of work item `it` (wave-uniform): the last frame with first <=
+// it.  One scalar load of its chunk's entry settles it unless a frame starts
+// inside the chunk past the chunk's first item; then the frame table is
+// scanned forward from the entry's frame.
+__device__ __forceinline__ int item_frame(const BatchArgs& a, uint32_t it) {
+    const uint2 e = a.chunk[it / kBatchChunk];
+    int fi = (int)e.x;
+    if (it >= e.y) {
+        ++fi;
+        while (fi + 1 < a.nframes && a.frames[fi + 1].first <= it) ++fi;
+    }
+    return fi;
+}
+
 // One work item `it` of a batch launch, as wave w of the workgroup.
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
 __device__ __forceinline__ void batch_item(const BatchArgs& a, uint32_t it, int w, uint8_t* L) {
     constexpr int GB = 4;
     using G = GroupLayout<TH, GB>;
-    // the item's frame: the last frame with first <= it, from the frame of
-    // its chunk's first item onward
-    int fi = a.chunk_frame[it / kBatchChunk];
-    while (fi + 1 < a.nframes && a.frames[fi + 1].first <= it) ++fi;
-    const BatchFrame& bf = a.frames[fi];
+    const BatchFrame& bf = a.frames[item_frame(a, it)];
     FastParams f{};
-    f.in = bf.in;
-    f.out = bf.out;
+    f.in = load_global_ptr(&bf.in);
+    f.out = load_global_ptr(&bf.out);
     f.in_bytes = bf.in_bytes;
     f.w = bf.w;
     f.h = bf.h;
@@ -61,10 +88,8 @@ __device__ __forceinline__ void batch_item(const BatchArgs& a, uint32_t it, int 
     f.wpb = GB;
     f.nsx = nsx;
     f.nby = nby;
-    const int local = (int)(it - bf.first);
-    const bool strip_fastest = (a.order & 2) != 0;
-    const int gy = strip_fastest ? local / nsx : local % nby;
-    const int sx = strip_fastest ? local % nsx : local / nby;
+    int sx, gy;
+    place_split(it - bf.first, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, gy);
     group_item<FMT, PAT, MODE, TH, GB, ODD, false, false>(f, strip_of(f, sx), gy * G::kRows, w, L, nullptr);
 }
 
@@ -101,10 +126,83 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
     }
 }
 
-// k_batch of one layout class over `blocks` workgroups of 4 waves.
+// Wave form: every wave is one work item, a strip by a band of TH rows
+// (items numbered frame by frame as nsx x nby bands), on the single-frame
+// kernel's per-wave path (run_band); no LDS is shared and no barrier taken.
+// Workgroups of one wave, placed like the group form's workgroups.
+template <int FMT, int PAT, int MODE, int TH, bool ODD>
+__device__ __forceinline__ void batch_wave_item(const BatchArgs& a, uint32_t it, uint32_t* stage) {
+    const BatchFrame& bf = a.frames[item_frame(a, it)];
+    FastParams f{};
+    f.in = load_global_ptr(&bf.in);
+    f.out = load_global_ptr(&bf.out);
+    f.in_bytes = bf.in_bytes;
+    f.w = bf.w;
+    f.h = bf.h;
+    f.bd = bf.bd;
+    f.maxv = bf.maxv;
+    f.clamp_hi = bf.clamp_hi;
+    f.pattern = PAT;
+    f.out16 = bf.flags & 1;
+    f.bl_on = (bf.flags >> 1) & 1;
+    for (int k = 0; k < 4; ++k) f.bl_i[k] = bf.bl_i[k];
+    const int nsx = bf.nsx, nby = bf.nby;
+    f.aligned = 1;
+    f.th = TH;
+    f.wpb = 1;
+    f.nsx = nsx;
+    f.nby = nby;
+    int sx, by;
+    place_split(it - bf.first, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, by);
+    const Strip sp = strip_of(f, sx);
+    const int y0 = by * TH;
+    const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
+    const bool hrim = sp.xs < 3 || sp.xs + 248 > f.w - 3;
+    if (vrim) run_band<FMT, PAT, MODE, true, TH, ODD, false, true>(f, sp, y0, stage);
+    else if (hrim) run_band<FMT, PAT, MODE, true, TH, ODD, false, false>(f, sp, y0, stage);
+    else run_band<FMT, PAT, MODE, false, TH, ODD, false, false>(f, sp, y0, stage);
+}
+
+template <int FMT, int PAT, int MODE, int TH, bool ODD, bool PERSIST>
+__global__ __launch_bounds__(64) void k_batch_w(BatchArgs a) {
+    __shared__ uint4 s_stage[128];  // one 1536-byte row slice plus the chunk reads' reach
+    uint32_t* stage = (uint32_t*)s_stage;
+    const int nblk = gridDim.x;
+    if (!PERSIST) {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        const int q8 = nblk >> 3, r8 = nblk & 7;
+        const int cnt = q8 + (xcd < r8 ? 1 : 0);
+        const int run = xcd >= 4 ? cnt - 1 - slot : slot;
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane(xcd * q8 + imin(xcd, r8) + run);
+        if (it < a.total) batch_wave_item<FMT, PAT, MODE, TH, ODD>(a, it, stage);
+        return;
+    }
+    const int S = nblk < 8 ? nblk : 8;
+    const int xcd = (int)blockIdx.x % S, slot = (int)blockIdx.x / S;
+    const uint32_t nx = (uint32_t)(nblk / S + (xcd < nblk % S ? 1 : 0));
+    const uint32_t lo = (uint32_t)((uint64_t)a.total * (uint32_t)xcd / (uint32_t)S);
+    const uint32_t hi = (uint32_t)((uint64_t)a.total * (uint32_t)(xcd + 1) / (uint32_t)S);
+    const bool rev = xcd >= 4;
+    for (uint32_t j = (uint32_t)slot; j < hi - lo; j += nx)
+        batch_wave_item<FMT, PAT, MODE, TH, ODD>(a, rev ? hi - 1u - j : lo + j, stage);
+}
+
+// k_batch of one layout class over `blocks` workgroups: wpb 4, group items
+// of 4 waves x rows (2 or 4); wpb 1, wave items of rows (4 or 8).
 template <int FMT, int PAT, int MODE, bool ODD>
 hipError_t launch_batch_class(const BatchArgs& a, int rows, bool persist, int blocks, hipStream_t st) {
     const dim3 grid(blocks), block(256);
+    if (a.wpb == 1) {
+        const dim3 wblock(64);
+        if (rows == 8) {
+            if (persist) hipLaunchKernelGGL((k_batch_w<FMT, PAT, MODE, 8, ODD, true>), grid, wblock, 0, st, a);
+            else hipLaunchKernelGGL((k_batch_w<FMT, PAT, MODE, 8, ODD, false>), grid, wblock, 0, st, a);
+        } else {
+            if (persist) hipLaunchKernelGGL((k_batch_w<FMT, PAT, MODE, 4, ODD, true>), grid, wblock, 0, st, a);
+            else hipLaunchKernelGGL((k_batch_w<FMT, PAT, MODE, 4, ODD, false>), grid, wblock, 0, st, a);
+        }
+        return hipGetLastError();
+    }
     if (rows == 4) {
         if (persist) hipLaunchKernelGGL((k_batch<FMT, PAT, MODE, 4, ODD, true>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_batch<FMT, PAT, MODE, 4, ODD, false>), grid, block, 0, st, a);

@@ -1028,9 +1028,7 @@ __device__ __forceinline__ void group_wave(const FastParams& f, const Strip& sp,
 // rows, whose groups run the slower per-wave rim path, start first instead
 // of finishing the launch (12 MP MG frame: 16.1 -> 15.8 us).  Strip-fastest
 // (bit 1): consecutive groups cover a group row left to right; else they
-// walk down a strip.  The quotient uses the host's reciprocal place_mul =
-// ceil(2^40 / divisor), exact while block count x divisor < 2^40 (the host
-// leaves place_mul 0 otherwise: plain division).
+// walk down a strip (place_split, with the host's place_mul).
 __device__ __forceinline__ void group_place(const FastParams& f, int& sx, int& g) {
     const int nblk = gridDim.x;
     // with nblk = 8q + r, XCD x runs cnt = q + (x < r) blocks: a bijection of [0, nblk)
@@ -1040,12 +1038,7 @@ __device__ __forceinline__ void group_place(const FastParams& f, int& sx, int& g
     const int run = xcd >= 4 ? cnt - 1 - slot : slot;
     const uint32_t lblk = (uint32_t)__builtin_amdgcn_readfirstlane(
         (f.xcd_remap & 1) ? xcd * q8 + imin(xcd, r8) + run : (int)blockIdx.x);
-    const bool strip_fastest = (f.xcd_remap & 2) != 0;
-    const uint32_t d = strip_fastest ? (uint32_t)f.nsx : (uint32_t)f.nby;
-    const uint32_t q = f.place_mul ? (uint32_t)(((uint64_t)lblk * f.place_mul) >> 40) : lblk / d;
-    const uint32_t r = lblk - q * d;
-    g = (int)(strip_fastest ? q : r);
-    sx = (int)(strip_fastest ? r : q);
+    place_split(lblk, (f.xcd_remap & 2) != 0, (uint32_t)f.nsx, (uint32_t)f.nby, f.place_mul, sx, g);
 }
 
 // Wave w (wave-uniform) of a group on the shared-LDS path.

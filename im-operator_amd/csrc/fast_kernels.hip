@@ -89,7 +89,7 @@ FastParams make_fast(const FrameParams& f, int th, int wpb, int xcd) {
     // (exact while blocks x divisor < 2^40; else 0: the kernel divides)
     const uint64_t d = (xcd & 2) ? (uint64_t)q.nsx : (uint64_t)q.nby;
     const uint64_t blocks = (uint64_t)q.nsx * (uint64_t)q.nby;
-    q.place_mul = blocks * d < (1ull << 40) ? ((1ull << 40) + d - 1) / d : 0;
+    q.place_mul = place_reciprocal(blocks, d);
     return q;
 }
 

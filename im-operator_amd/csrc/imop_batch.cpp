@@ -64,10 +64,14 @@ int check_frame(const imop_hip_frame_desc& d) {
 
 namespace {
 
-// Launch shape of k_batch (imop_hip_set_batch_shape): rows per wave and
-// waves per workgroup of the group items, workgroups per CU of a persistent
-// grid (0: one workgroup per item), order (bit 1: strip-fastest).
-std::atomic<int> g_rows{2}, g_wpb{4}, g_bpc{0}, g_order{2};
+// Launch shape of k_batch (imop_hip_set_batch_shape): waves per workgroup
+// (4: group items of 4 waves x rows, rows 2 or 4; 1: wave items of rows, 4 or
+// 8), workgroups per CU of a persistent grid (0: one workgroup per item),
+// order (bit 1: strip-fastest).
+std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
+
+// Rows per wave the form runs: group items take 2 or 4, wave items 4 or 8.
+int batch_rows(int rows, int wpb) { return wpb == 1 ? (rows == 8 ? 8 : 4) : (rows == 2 ? 2 : 4); }
 
 int cu_count() {
     static std::mutex mu;
@@ -90,15 +94,15 @@ struct Group {
     bool odd = false;
     std::vector<BatchFrame> frames;
     uint32_t total = 0;
-    std::vector<uint16_t> chunk_frame;  // frame of item c * kBatchChunk
+    std::vector<uint32_t> chunk;  // {frame of item c * kBatchChunk, first item of the next frame} by chunk c
 
     // The device copy: the frame table, then the chunk table at a 16-byte boundary.
     size_t chunk_offset() const { return (frames.size() * sizeof(BatchFrame) + 15u) & ~(size_t)15u; }
-    size_t device_bytes() const { return chunk_offset() + chunk_frame.size() * sizeof(uint16_t); }
+    size_t device_bytes() const { return chunk_offset() + chunk.size() * sizeof(uint32_t); }
     std::vector<uint8_t> device_image() const {
         std::vector<uint8_t> b(device_bytes(), 0);
         std::memcpy(b.data(), frames.data(), frames.size() * sizeof(BatchFrame));
-        std::memcpy(b.data() + chunk_offset(), chunk_frame.data(), chunk_frame.size() * sizeof(uint16_t));
+        std::memcpy(b.data() + chunk_offset(), chunk.data(), chunk.size() * sizeof(uint32_t));
         return b;
     }
 };
@@ -112,7 +116,7 @@ struct Prepared {
     std::vector<Group> groups;
     std::vector<Single> singles;
     size_t prism_ws = 0;
-    int rows = 2, wpb = 4, bpc = 0, order = 2;
+    int rows = 4, wpb = 4, bpc = 0, order = 2;
 };
 
 int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
@@ -126,8 +130,8 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
             return e;
         }
     }
-    P.rows = g_rows;
     P.wpb = g_wpb;
+    P.rows = batch_rows(g_rows, P.wpb);
     P.bpc = g_bpc;
     P.order = g_order;
     std::map<int, size_t> index;  // group of each launch key
@@ -137,7 +141,8 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         FrameParams f = make_params(d.in, d.in_bytes, d.out, d.width, d.height, d.pattern, d.bit_depth, d.is_packed,
                                     d.black_level, d.black_level_mode, d.algorithm, &mode);
         f.out16 = out_elem_of(d) == 2;
-        // the single-frame kernel's items: aligned strips for u16 RGB, group rows of rows x wpb
+        // the single-frame kernel's items: aligned strips for u16 RGB by
+        // group rows of rows x 4 (group form) or bands of rows (wave form)
         const int nsx = strip_count(d.width, f.out16 != 0);
         const int nby = (d.height + P.rows * P.wpb - 1) / (P.rows * P.wpb);
         // k_batch runs MG 8-bit frames on the integer path; a packed 8-bit
@@ -165,7 +170,7 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         }
         Group& g = P.groups[it->second];
         const uint64_t items = (uint64_t)nsx * (uint64_t)nby;
-        // 31-bit item numbers; frame numbers fit the 16-bit chunk table
+        // 31-bit item numbers; at most 65535 frames per launch
         if ((uint64_t)g.total + items >= (1ull << 31) || g.frames.size() >= 65535u) return kInternal;
         BatchFrame b{};
         b.in = f.in;
@@ -182,17 +187,19 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         b.nsx = (uint16_t)nsx;
         b.nby = (uint16_t)nby;
         b.first = g.total;
+        b.place_mul = place_reciprocal(items, (P.order & 2) ? (uint64_t)nsx : (uint64_t)nby);
         g.frames.push_back(b);
         g.total += (uint32_t)items;
     }
     for (Group& g : P.groups) {
         const uint32_t nchunks = (g.total + kBatchChunk - 1) / kBatchChunk;
-        g.chunk_frame.resize(nchunks);
+        g.chunk.resize(2 * (size_t)nchunks);
         size_t fi = 0;
         for (uint32_t c = 0; c < nchunks; ++c) {
             const uint32_t item = c * (uint32_t)kBatchChunk;
             while (fi + 1 < g.frames.size() && g.frames[fi + 1].first <= item) ++fi;
-            g.chunk_frame[c] = (uint16_t)fi;
+            g.chunk[2 * c] = (uint32_t)fi;
+            g.chunk[2 * c + 1] = fi + 1 < g.frames.size() ? g.frames[fi + 1].first : g.total;
         }
     }
     return kOk;
@@ -207,7 +214,7 @@ int enqueue(const Prepared& P, const std::vector<void*>& tables, void* prism_ws,
         if (g.total == 0) continue;
         BatchArgs a;
         a.frames = (const BatchFrame*)tables[i];
-        a.chunk_frame = (const uint16_t*)((const uint8_t*)tables[i] + g.chunk_offset());
+        a.chunk = (const uint2*)((const uint8_t*)tables[i] + g.chunk_offset());
         a.nframes = (int)g.frames.size();
         a.total = g.total;
         a.wpb = P.wpb;
@@ -345,8 +352,8 @@ int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan) {
 }
 
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order) {
-    if (rows_per_wave == 2 || rows_per_wave == 4) g_rows = rows_per_wave;
-    if (waves_per_block == 4) g_wpb = waves_per_block;
+    if (rows_per_wave == 2 || rows_per_wave == 4 || rows_per_wave == 8) g_rows = rows_per_wave;
+    if (waves_per_block == 1 || waves_per_block == 4) g_wpb = waves_per_block;
     if (blocks_per_cu >= 0 && blocks_per_cu <= 64) g_bpc = blocks_per_cu;
     if (order >= 0 && order <= 3) g_order = order & 2;
     return kOk;

@@ -183,13 +183,16 @@ int imop_hip_batch_plan_create(const imop_hip_frame_desc* frames, int n, imop_hi
 int imop_hip_batch_plan_launch(imop_hip_batch_plan* plan, void* stream);
 int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan);
 
-/* Launch shape of the batched kernel, chosen by the autotuner.  Its work
- * items are the single-frame kernel's workgroups: rows each wave computes
- * (2 or 4) times waves per workgroup stacked down one strip (4);
- * workgroups per CU of the persistent grid (1..64; 0 = one workgroup per
- * item); order (bit 1: strip-fastest within a frame).  Initial shape
- * (2, 4, 0, 2).  Out-of-range values leave a setting unchanged.
- * Process-wide; no counterpart in the reference. */
+/* Launch shape of the batched kernel, chosen by the autotuner.
+ * waves_per_block 4: work items are the single-frame kernel's workgroups,
+ * 4 waves stacked down one strip sharing decoded rows through LDS, each wave
+ * computing rows_per_wave rows (2 or 4); waves_per_block 1: every wave is
+ * its own item, a strip by a band of rows_per_wave rows (4 or 8).  A rows
+ * value the form does not take runs as its nearest (group form: 8 -> 4;
+ * wave form: 2 -> 4).  blocks_per_cu: workgroups per CU of the persistent
+ * grid (1..64; 0 = one workgroup per item); order: bit 1 strip-fastest
+ * within a frame.  Initial shape (4, 4, 0, 2).  Out-of-range values leave a
+ * setting unchanged.  Process-wide; no counterpart in the reference. */
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order);
 int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order);
 

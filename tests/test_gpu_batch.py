@@ -104,8 +104,7 @@ def test_ragged_batch_plan_replays_match_oracle():
             _check_against_oracle(frames)
 
 
-@pytest.mark.parametrize("rows", [2, 4])
-@pytest.mark.parametrize("wpb", [4])
+@pytest.mark.parametrize("rows,wpb", [(2, 4), (4, 4), (4, 1), (8, 1)])
 @pytest.mark.parametrize("bpc", [0, 1, 3, 16])
 @pytest.mark.parametrize("order", [0, 2])
 def test_every_batch_shape(rows, wpb, bpc, order, restore_batch_shape):
