@@ -72,6 +72,8 @@ def golden_sha(name):
 # plain device copy of known size gives in the same run)
 TRAFFIC_PROFILES = {
     "mg12": "profiles/k_fast_mg12_hbm_traffic.md",
+    "batch": "profiles/k_batch_mixed64_trace_and_traffic.md",
+    "prism10": "profiles/prism10_trace_traffic_counters.md",
 }
 
 
@@ -363,7 +365,8 @@ def bench_batch(steps, warmup, with_cpu, shape=None):
         "autotune_table_ms": autotune_table,
         "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (ms_plan / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg_bytes / (ms_plan / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes_per_step": alg_bytes},
+                     "algorithmic_bytes_per_step": alg_bytes, "traffic": measured_traffic("batch")[0],
+                     "traffic_source": measured_traffic("batch")[1]},
         "parity": parity,
         "steps": steps,
         "warmup": warmup,
@@ -443,6 +446,9 @@ def bench_single(name, steps, warmup, sets=4, graph=False, with_cpu=False):
     log(f"bench: {name} {ms * 1e3:.1f} us per frame, parity {parity}")
     hbm = {"bound": "hbm", "algorithmic_bytes_per_frame": alg, "achieved": round(alg / (ms / 1e3) / 1e9, 1),
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    traffic, traffic_src = measured_traffic(name)
+    if traffic is not None:
+        hbm.update({"traffic": traffic, "traffic_source": traffic_src})
     out = {"workload": name, "value": round(w * h / (ms / 1e3) / 1e6, 1), "unit": "Mpix/s",
            "ms_per_frame": round(ms, 5), "parity": parity, "timing": "graph replay" if graph else "eager launches"}
     out.update(binding_roofline(hbm, valu_roofline(name, ms / 1e3)))
