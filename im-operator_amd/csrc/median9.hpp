@@ -27,4 +27,35 @@ __device__ __forceinline__ float median9(float v[9]) {
     return v[4];
 }
 
+// The network's first nine exchanges sort each row triple (v[0..2], v[3..5],
+// v[6..8]) on its own, by (0,1) (1,2) (0,1); they touch disjoint triples, so
+// a sorted triple can be formed once and shared by the two pixels whose
+// windows hold that row.  sort3 is that triple sort; median9_of_sorted_rows
+// runs the remaining fourteen exchanges in the network's order, so
+// median9_of_sorted_rows(rows sorted by sort3) == median9(rows) bit for bit.
+__device__ __forceinline__ void sort3(float& a, float& b, float& c) {
+    float t = fminf(a, b);
+    b = fmaxf(a, b);
+    a = t;
+    t = fminf(b, c);
+    c = fmaxf(b, c);
+    b = t;
+    t = fminf(a, b);
+    b = fmaxf(a, b);
+    a = t;
+}
+
+__device__ __forceinline__ float median9_of_sorted_rows(float v[9]) {
+#define IMOP_CX(a, b)                        \
+    {                                        \
+        const float ta = v[a], tb = v[b];    \
+        v[a] = fminf(ta, tb);                \
+        v[b] = fmaxf(ta, tb);                \
+    }
+    IMOP_CX(0, 3) IMOP_CX(4, 7) IMOP_CX(1, 4) IMOP_CX(3, 6) IMOP_CX(2, 5) IMOP_CX(5, 8) IMOP_CX(1, 3)
+    IMOP_CX(4, 6) IMOP_CX(2, 5) IMOP_CX(2, 3) IMOP_CX(5, 6) IMOP_CX(2, 4) IMOP_CX(3, 5) IMOP_CX(3, 4)
+#undef IMOP_CX
+    return v[4];
+}
+
 }  // namespace imop_hip

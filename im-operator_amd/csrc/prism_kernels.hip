@@ -453,31 +453,47 @@ __global__ __launch_bounds__(256) void k_prism_b(FrameParams f, void* ws) {
     // s_pq, B into s_vh (both dead now), tile coordinates
     float* s_r9 = &s_pq[0][0];
     float* s_b9 = &s_vh[0][0];
-    if (!RCD)
-    for (int k = threadIdx.x; k < TH * TW; k += blockDim.x) {
-        const int j = k / TW, i = k - j * TW;
-        const int x = x0 + i, y = y0 + j;
-        const int rj = j + 4, ri = i + 4, gj = j + 6, gi = i + 6;
-        float r9 = s_r[rj][ri], b9 = s_b[rj][ri];
-        if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
-            float a[9], bb[9];
-            int q = 0;
+    // Each thread takes two vertically adjacent pixels of one column (tile
+    // rows j0, j0 + 1): the four row triples of colour differences their
+    // windows span are formed and sorted once, the middle two shared.
+    if (!RCD) {
+        static_assert(TW == 32 && TH == 16, "S7-S9: 256 threads x 2 rows cover the tile");
+        const int i = threadIdx.x & (TW - 1), j0 = (threadIdx.x / TW) * 2;
+        const int x = x0 + i;
+        float tr[4][3], tb[4][3];
 #pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
+        for (int q = 0; q < 4; ++q) {
+            const int jj = j0 - 1 + q;
 #pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const float g = s_g[gj + dy][gi + dx];
-                    a[q] = s_r[rj + dy][ri + dx] - g;
-                    bb[q] = s_b[rj + dy][ri + dx] - g;
-                    ++q;
-                }
-            const float lm = median9(a), mm = median9(bb);
-            const float g0 = s_g[gj][gi];
-            r9 = clamp01(g0 + lm);
-            b9 = clamp01(g0 + mm);
+            for (int dx = -1; dx <= 1; ++dx) {
+                const float g = s_g[jj + 6][i + 6 + dx];
+                tr[q][dx + 1] = s_r[jj + 4][i + 4 + dx] - g;
+                tb[q][dx + 1] = s_b[jj + 4][i + 4 + dx] - g;
+            }
+            sort3(tr[q][0], tr[q][1], tr[q][2]);
+            sort3(tb[q][0], tb[q][1], tb[q][2]);
         }
-        s_r9[k] = r9;
-        s_b9[k] = b9;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int j = j0 + p, y = y0 + j;
+            float r9 = s_r[j + 4][i + 4], b9 = s_b[j + 4][i + 4];
+            if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
+                float a[9], bb[9];
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        a[3 * q + c] = tr[p + q][c];
+                        bb[3 * q + c] = tb[p + q][c];
+                    }
+                const float lm = median9_of_sorted_rows(a), mm = median9_of_sorted_rows(bb);
+                const float g0 = s_g[j + 6][i + 6];
+                r9 = clamp01(g0 + lm);
+                b9 = clamp01(g0 + mm);
+            }
+            s_r9[j * TW + i] = r9;
+            s_b9[j * TW + i] = b9;
+        }
     }
     __syncthreads();
     // S10 + S11: width-4 border fill by source remapping (the source is in
