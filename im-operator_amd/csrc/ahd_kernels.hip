@@ -21,6 +21,7 @@
 
 #include "device_common.hpp"
 #include "median9.hpp"
+#include "window_decode.hpp"
 #include "plain.hpp"
 
 #pragma clang fp contract(off)
@@ -67,11 +68,14 @@ __global__ __launch_bounds__(256) void k_ahd(FrameParams f) {
     const int ntx = (w + TW - 1) / TW;
     const int x0 = (blockIdx.x % ntx) * TW, y0 = (blockIdx.x / ntx) * TH;
 
-    for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
-        const int j = k / RW, i = k - j * RW;
-        const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
-        s_raw[j][i] = (float)sample_at(f, x, y);
-    }
+    // samples at clamped positions; a window inside the frame (all but the
+    // edge tiles) needs no clamp and takes the per-layout rows
+    if (!window_inside<RW, RH>(f, x0 - 3, y0 - 3, [&](int j, int i, int v) { s_raw[j][i] = (float)v; }))
+        for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
+            const int j = k / RW, i = k - j * RW;
+            const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
+            s_raw[j][i] = (float)sample_at(f, x, y);
+        }
     __syncthreads();
     // unfiltered differences at frame positions x0-2+i, y0-2+j (in-frame ones are read)
     for (int k = threadIdx.x; k < DH * DW; k += blockDim.x) {
@@ -90,23 +94,37 @@ __global__ __launch_bounds__(256) void k_ahd(FrameParams f) {
     }
     __syncthreads();
     // filtered differences at x0-1+i, y0-1+j: the network inside [1, w-1) x [1, h-1), the value on the ring
-    for (int k = threadIdx.x; k < FH * FW; k += blockDim.x) {
-        const int j = k / FW, i = k - j * FW;
-        const int x = x0 - 1 + i, y = y0 - 1 + j;
-        if (x < 0 || x >= w || y < 0 || y >= h) continue;
-        const bool inner = x >= 1 && x < w - 1 && y >= 1 && y < h - 1;
+    // Each task is two vertically adjacent positions of one column (rows j0,
+    // j0 + 1): the four row triples their windows span are sorted once, the
+    // middle two shared (median9_of_sorted_rows).  A triple that reaches
+    // outside the frame holds unwritten entries and is used by no position.
+    static_assert(FH % 2 == 0, "AHD filter: row pairs");
+    for (int t = threadIdx.x; t < (FH / 2) * FW; t += blockDim.x) {
+        const int jp = t / FW, i = t - jp * FW, j0 = 2 * jp;
+        const int x = x0 - 1 + i;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            if (inner) {
-                float v[9];
-                int q = 0;
+            float tr[4][3];
 #pragma unroll
-                for (int dy = -1; dy <= 1; ++dy)
+            for (int q = 0; q < 4; ++q) {
 #pragma unroll
-                    for (int dx = -1; dx <= 1; ++dx) v[q++] = s_d[p][j + 1 + dy][i + 1 + dx];
-                s_f[p][j][i] = median9(v);
-            } else {
-                s_f[p][j][i] = s_d[p][j + 1][i + 1];
+                for (int dx = -1; dx <= 1; ++dx) tr[q][dx + 1] = s_d[p][j0 + q][i + 1 + dx];
+                sort3(tr[q][0], tr[q][1], tr[q][2]);
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int j = j0 + r, y = y0 - 1 + j;
+                if (x < 0 || x >= w || y < 0 || y >= h) continue;
+                if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
+                    float v[9];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) v[3 * q + c] = tr[r + q][c];
+                    s_f[p][j][i] = median9_of_sorted_rows(v);
+                } else {
+                    s_f[p][j][i] = s_d[p][j + 1][i + 1];
+                }
             }
         }
     }

@@ -31,6 +31,7 @@
 #include "device_common.hpp"
 #include "median9.hpp"
 #include "plain.hpp"
+#include "window_decode.hpp"
 
 #pragma clang fp contract(off)
 
@@ -219,61 +220,15 @@ __device__ __forceinline__ float norm_sample(int v, float m, float y) {
     return fmaf(fmaf(-m, q0, a), y, q0);
 }
 
-// Raw sample at pixel index i of a window that lies inside the frame and
-// above its last row: the decoders of sample_at without the stream-end
-// bound, which only the last row's final group can reach, in 32-bit offsets
-// (the passes take frames of fewer than 2^31 pixels).
-template <int FMT>
-__device__ __forceinline__ int raw_inside(const uint8_t* __restrict__ d, uint32_t i) {
-    if (FMT == FMT_U8) return d[i];
-    if (FMT == FMT_U16) return ((const uint16_t*)d)[i];
-    if (FMT == FMT_P12) {
-        const uint32_t o = (i >> 1) * 3u;
-        const unsigned b1 = d[o + 1];
-        return (i & 1u) == 0 ? (int)(((unsigned)d[o] << 4) | (b1 & 0xFu)) : (int)(((unsigned)d[o + 2] << 4) | (b1 >> 4));
-    }
-    const uint32_t o = (i >> 2) * 5u, sub = i & 3u;  // FMT_P10
-    return (int)(((unsigned)d[o + sub] << 2) | (((unsigned)d[o + 4] >> (2u * sub)) & 3u));
-}
-
-// S0 rows of the window: lane c takes column xs + c of rows wv, wv + 4, ...
-// (wave wv); sample_at's black level and u16 MSB alignment, with the lane's
-// two black-level offsets (even and odd rows) chosen once.
-template <int FMT, int RW, int RH>
-__device__ __forceinline__ void s0_inside(const FrameParams& f, float* dst, int xs, int ys, float scale,
-                                          float rscale) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane >= RW) return;
-    const int x = xs + lane;
-    const bool xodd = (x & 1) != 0;
-    const float bl_even = xodd ? f.bl_q[1] : f.bl_q[0], bl_odd = xodd ? f.bl_q[3] : f.bl_q[2];
-    const bool msb = FMT == FMT_U16 && f.bd > 8 && f.bd < 16;
-    for (int j = wv; j < RH; j += 4) {
-        const int y = ys + j;
-        int v = raw_inside<FMT>(f.in, (uint32_t)y * (uint32_t)f.w + (uint32_t)x);
-        if (f.bl_on) v = iclamp((int)((float)v - ((y & 1) ? bl_odd : bl_even)), 0, f.maxv);
-        else if (msb && v > f.maxv) v >>= (16 - f.bd);
-        dst[j * RW + lane] = norm_sample(v, scale, rscale);
-    }
-}
-
 // S0 over the RW x RH window whose top-left frame position is (xs, ys), 0
 // outside the frame: windows inside the frame and above its last row take
-// the per-layout rows above; the others, and the GigE and generic bit
-// layouts, sample_at per pixel.
+// the per-layout rows of window_inside; the others, and the GigE and generic
+// bit layouts, sample_at per pixel.
 template <int RW, int RH>
 __device__ __forceinline__ void s0_window(const FrameParams& f, float* dst, int xs, int ys, float scale, float rscale) {
-    static_assert(RW <= 64, "S0: one row per wave step");
+    if (window_inside<RW, RH>(f, xs, ys, [&](int j, int c, int v) { dst[j * RW + c] = norm_sample(v, scale, rscale); }))
+        return;
     const int w = f.w, h = f.h;
-    if (xs >= 0 && ys >= 0 && xs + RW <= w && ys + RH < h) {  // workgroup-uniform
-        switch (f.fmt) {
-        case FMT_U8: s0_inside<FMT_U8, RW, RH>(f, dst, xs, ys, scale, rscale); return;
-        case FMT_U16: s0_inside<FMT_U16, RW, RH>(f, dst, xs, ys, scale, rscale); return;
-        case FMT_P12: s0_inside<FMT_P12, RW, RH>(f, dst, xs, ys, scale, rscale); return;
-        case FMT_P10: s0_inside<FMT_P10, RW, RH>(f, dst, xs, ys, scale, rscale); return;
-        default: break;
-        }
-    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane >= RW) return;
     const int x = xs + lane;
