@@ -272,20 +272,28 @@ int imop_hip_demosaic_batch(const imop_hip_frame_desc* frames, int n, void* stre
     // replay would read after the call returned: graphs take the plan form.
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return kInternal;
-    // work lists live in stream-ordered allocations released after the launch
-    std::vector<void*> tables(P.groups.size(), nullptr);
-    std::vector<std::vector<uint8_t>> images(P.groups.size());
+    // every launch's work list in one stream-ordered allocation (one copy,
+    // each list at a 256-byte boundary), released after the launches
+    std::vector<size_t> offset(P.groups.size(), 0);
+    size_t bytes = 0;
     for (size_t i = 0; i < P.groups.size(); ++i) {
-        images[i] = P.groups[i].device_image();
-        if (hipMallocAsync(&tables[i], images[i].size(), st) != hipSuccess ||
-            hipMemcpyAsync(tables[i], images[i].data(), images[i].size(), hipMemcpyHostToDevice, st) != hipSuccess) {
-            e = kInternal;
-            break;
-        }
+        offset[i] = bytes;
+        bytes += (P.groups[i].device_bytes() + 255u) & ~(size_t)255u;
     }
+    std::vector<void*> tables(P.groups.size(), nullptr);
+    if (bytes == 0) return enqueue(P, tables, nullptr, st);
+    // kept alive until the call returns: the copy reads it in stream order
+    std::vector<uint8_t> image(bytes, 0);
+    for (size_t i = 0; i < P.groups.size(); ++i) {
+        const std::vector<uint8_t> g = P.groups[i].device_image();
+        std::memcpy(image.data() + offset[i], g.data(), g.size());
+    }
+    void* base = nullptr;
+    if (hipMallocAsync(&base, bytes, st) != hipSuccess) return kInternal;
+    if (hipMemcpyAsync(base, image.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) e = kInternal;
+    for (size_t i = 0; i < P.groups.size(); ++i) tables[i] = (uint8_t*)base + offset[i];
     if (e == kOk) e = enqueue(P, tables, nullptr, st);
-    for (void* t : tables)
-        if (t) (void)hipFreeAsync(t, st);
+    (void)hipFreeAsync(base, st);
     return e;
 }
 
