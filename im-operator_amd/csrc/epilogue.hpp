@@ -35,9 +35,38 @@ __device__ __forceinline__ int epi_round_clamp(float v, int maxv) {
 }
 
 // One pixel through the stages; `look(v)` is the gamma table lookup.
+// BOUNDED keeps the values in float between the stages: a stage's result
+// (int)med3(v, 0, max) enters the next as its float, which is
+// truncf(med3(v, 0, max)) exactly (integers up to 65535).
 template <bool BOUNDED, typename Look>
 __device__ __forceinline__ void epi_px_t(const Epilogue& e, int& r, int& g, int& b, Look look) {
 #pragma clang fp contract(off)
+    if (BOUNDED && (e.flags & (EPI_WB | EPI_CCM))) {
+        const float mx = (float)e.maxv;
+        float fr = (float)r, fg = (float)g, fb = (float)b;
+        if (e.flags & EPI_WB) {
+            fr = truncf(__builtin_amdgcn_fmed3f(fmaf(fr, e.gain[0], 0.5f), 0.0f, mx));
+            fg = truncf(__builtin_amdgcn_fmed3f(fmaf(fg, e.gain[1], 0.5f), 0.0f, mx));
+            fb = truncf(__builtin_amdgcn_fmed3f(fmaf(fb, e.gain[2], 0.5f), 0.0f, mx));
+        }
+        if (e.flags & EPI_CCM) {
+            const float ro = fmaf(e.m[2], fb, fmaf(e.m[0], fr, e.m[1] * fg));
+            const float go = fmaf(e.m[5], fb, fmaf(e.m[3], fr, e.m[4] * fg));
+            const float bo = fmaf(e.m[8], fb, fmaf(e.m[6], fr, e.m[7] * fg));
+            fr = __builtin_amdgcn_fmed3f(ro + 0.5f, 0.0f, mx);
+            fg = __builtin_amdgcn_fmed3f(go + 0.5f, 0.0f, mx);
+            fb = __builtin_amdgcn_fmed3f(bo + 0.5f, 0.0f, mx);
+        }
+        r = (int)fr;
+        g = (int)fg;
+        b = (int)fb;
+        if (e.flags & EPI_GAMMA) {
+            r = look(r);
+            g = look(g);
+            b = look(b);
+        }
+        return;
+    }
     if (e.flags & EPI_WB) {
         r = epi_round_clamp<BOUNDED>(fmaf((float)r, e.gain[0], 0.5f), e.maxv);
         g = epi_round_clamp<BOUNDED>(fmaf((float)g, e.gain[1], 0.5f), e.maxv);

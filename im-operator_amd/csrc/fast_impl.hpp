@@ -1087,9 +1087,13 @@ __global__ __launch_bounds__(64 * GB) void k_fast(FastParams f) {
     // epilogue form: the part of the gamma table the demosaic's values can
     // reach (lut_last + 1 <= 4096 entries, checked on the host) in LDS,
     // filled by the whole workgroup before any wave leaves
-    __shared__ uint16_t s_lut[EPI ? 4096 : 1];
+    __shared__ __align__(16) uint16_t s_lut[EPI ? 4096 : 8];
     if (EPI && (f.epi.flags & EPI_GAMMA)) {
-        for (int i = (int)threadIdx.x; i <= (int)lut_last(f); i += (int)blockDim.x) s_lut[i] = f.epi.lut[i];
+        // in 16-byte pieces: entries 0 .. lut_last rounded up to a multiple of
+        // 8, inside the table (256 or 65536 entries, 256-byte aligned)
+        const int n16 = ((int)lut_last(f) + 8) >> 3;
+        const uint4* __restrict__ src = (const uint4*)f.epi.lut;
+        for (int i = (int)threadIdx.x; i < n16; i += (int)blockDim.x) ((uint4*)s_lut)[i] = src[i];
         __syncthreads();
     }
     int sx, g;
