@@ -37,6 +37,7 @@ namespace {
 constexpr int kTW = 64;   // backward tile: 64 x 16 raw pixels per iteration
 constexpr int kTH = 16;
 constexpr int kFW = 256;  // forward: row segments of 256 pixels per workgroup
+constexpr int kFR = 4;    // forward: rows per step (at most the 6-row minimum frame height)
 
 __device__ __forceinline__ bool interior(int x, int y, int w, int h) {
     return x >= 2 && x < w - 2 && y >= 2 && y < h - 2;
@@ -171,38 +172,79 @@ __device__ __forceinline__ Params load_params(const float* gains, const float* c
 
 __global__ __launch_bounds__(256) void k_chain_fwd(ChainArgs A) {
     // workgroup (bx, by): columns bx * 256 + tid of rows [by * per, (by + 1) * per)
-    // of the n * h rows of the batch
+    // of the n * h rows of the batch, a contiguous run, so the 3-row
+    // neighbourhoods are re-read from the caches rather than from HBM
     const Params P = load_params(A.gains, A.ccm);
     const int w = A.w, h = A.h;
     const int x = blockIdx.x * kFW + threadIdx.x;
     const float im = A.inv_max;
     float acc[1] = {0.0f};
-    // a contiguous run of rows per workgroup, so the 3-row neighbourhoods are
-    // re-read from the caches rather than from HBM
     const int rows = A.n * h;
     const int per = (rows + gridDim.y - 1) / gridDim.y;
-    const int r1 = min(rows, (int)(blockIdx.y + 1) * per);
-    for (int ry = blockIdx.y * per; ry < r1; ++ry) {
-        if (x >= w) continue;
-        const int f = ry / h, y = ry - f * h;
-        const int64_t i = (int64_t)ry * w + x;  // pixel index in the batch
-        float a[3] = {0.0f, 0.0f, 0.0f};
-        int sx, sy;
-        if (src_of(x, y, w, h, sx, sy)) {
-            const float* c = A.raw + ((int64_t)f * h + sy) * w + sx;
-            bilinear_at(A.pattern, sx, sy, [&](int dx, int dy) { return c[dy * w + dx] * im; }, a);
-        }
-        Px s;
-        chain_px(a, P.g, P.M, A.inv_gamma, s);
-        if (A.out) {
+    const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+    if (x < w && r0 < r1) {
+        // fill_rgb_borders' source column (every frame here is at least 6 x 6)
+        const int sx = min(max(x, 2), w - 3);
+        const bool xin = x >= 2 && x < w - 2;
+        int f = r0 / h, y = r0 - f * h;  // frame and row of row ry
+        // kFR rows a step, every load of the step issued before any of its
+        // arithmetic, so a wave waits on memory once per step, not per row
+        for (int ry = r0; ry < r1; ry += kFR) {
+            float nb[kFR][9], tg[kFR][3];
+            int ys[kFR];
+            bool on[kFR];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) A.out[i * 3 + k] = s.y[k];
-        }
-        if (A.target) {
+            for (int u = 0; u < kFR; ++u) {
+                int yu = y + u, fu = f;
+                if (yu >= h) {  // kFR <= h: at most one frame boundary a step
+                    yu -= h;
+                    ++fu;
+                }
+                // rows past the run load the run's last row again and are dropped
+                if (ry + u >= r1) {
+                    yu = (r1 - 1) - ((r1 - 1) / h) * h;
+                    fu = (r1 - 1) / h;
+                }
+                ys[u] = yu;
+                on[u] = xin || (yu >= 2 && yu < h - 2);  // not one of the 16 corner pixels
+                const int sy = min(max(yu, 2), h - 3);
+                const float* c = A.raw + ((int64_t)fu * h + sy) * w + sx;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float d = s.y[k] - A.target[i * 3 + k];
-                acc[0] += d * d;
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) nb[u][(dy + 1) * 3 + dx + 1] = c[dy * w + dx];
+                if (A.target) {
+                    const int64_t i = ((int64_t)fu * h + yu) * w + x;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) tg[u][k] = A.target[i * 3 + k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kFR; ++u) {
+                if (ry + u >= r1) break;
+                const int yu = ys[u];
+                const int sy = min(max(yu, 2), h - 3);
+                float a[3] = {0.0f, 0.0f, 0.0f};
+                if (on[u]) bilinear_at(A.pattern, sx, sy, [&](int dx, int dy) { return nb[u][(dy + 1) * 3 + dx + 1] * im; }, a);
+                Px s;
+                chain_px(a, P.g, P.M, A.inv_gamma, s);
+                const int64_t i = (int64_t)(ry + u) * w + x;  // pixel index in the batch
+                if (A.out) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) A.out[i * 3 + k] = s.y[k];
+                }
+                if (A.target) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float d = s.y[k] - tg[u][k];
+                        acc[0] += d * d;
+                    }
+                }
+            }
+            y += kFR;
+            if (y >= h) {
+                y -= h;
+                ++f;
             }
         }
     }
