@@ -72,14 +72,25 @@ constexpr uint32_t kAnalyzeOps = OP_MAX | OP_SUM << 2 | OP_SUM << 4 | OP_MAX << 
                                  OP_MIN << 12 | OP_MIN << 14 | OP_MAX << 16 | OP_MAX << 18 | OP_MAX << 20 |
                                  OP_MAX << 22 | OP_MIN << 24 | OP_MAX << 26 | OP_MIN << 28 | OP_MAX << 30;
 
-__device__ __forceinline__ void minmax_word(uint32_t w, unsigned long long (&v)[16]) {
-    const uint32_t lo = w & 0xFFFFu, hi = w >> 16;
-    const uint32_t b0 = w & 0xFFu, b1 = (w >> 8) & 0xFFu, b2 = (w >> 16) & 0xFFu, b3 = w >> 24;
-    v[12] = min(v[12], (unsigned long long)min(lo, hi));
-    v[13] = max(v[13], (unsigned long long)max(lo, hi));
-    v[14] = min(v[14], (unsigned long long)min(min(b0, b1), min(b2, b3)));
-    v[15] = max(v[15], (unsigned long long)max(max(b0, b1), max(b2, b3)));
+// Running minima / maxima of the whole-buffer scan as packed 16-bit pairs:
+// the word's two u16 values, and its four bytes as two pairs (even and odd
+// bytes widened to 16 bits), one packed min / max instruction each.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+struct ScanAcc {
+    u16x2 mn16, mx16, mn8, mx8;
+};
+
+__device__ __forceinline__ void scan_word(uint32_t w, ScanAcc& a) {
+    const u16x2 x = __builtin_bit_cast(u16x2, w);
+    a.mn16 = __builtin_elementwise_min(a.mn16, x);
+    a.mx16 = __builtin_elementwise_max(a.mx16, x);
+    const u16x2 e = __builtin_bit_cast(u16x2, w & 0x00FF00FFu), o = __builtin_bit_cast(u16x2, (w >> 8) & 0x00FF00FFu);
+    a.mn8 = __builtin_elementwise_min(a.mn8, __builtin_elementwise_min(e, o));
+    a.mx8 = __builtin_elementwise_max(a.mx8, __builtin_elementwise_max(e, o));
 }
+
+// Steps of kScanU 16-byte chunks per thread, all loads of a step issued first.
+constexpr int kScanU = 4;
 
 
 // All statistics analyze_data reads, one row per workgroup (reduced by
@@ -89,8 +100,8 @@ __device__ __forceinline__ void minmax_word(uint32_t w, unsigned long long (&v)[
 //     sample[k] pixels at depths 10, 12, 14, 16 (at most 100000 values);
 //   - with scan, the whole buffer's range as u16 and as bytes (the unpacked
 //     16- and 8-bit branches, analyzer.cpp:163-170, 195-202): 16-byte chunks,
-//     two in flight per thread, when d is 16-byte aligned, then the tail value
-//     by value.
+//     kScanU in flight per thread, when d is 16-byte aligned, then the tail
+//     value by value.
 __global__ __launch_bounds__(kThreads) void k_analyze(const uint8_t* __restrict__ d, uint64_t n, uint64_t n1,
                                                       uint64_t n2, AnalyzeParams p, int scan, AnalyzeRow* rows) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -121,16 +132,32 @@ __global__ __launch_bounds__(kThreads) void k_analyze(const uint8_t* __restrict_
         const uint64_t nv = ((uintptr_t)d & 15u) == 0 ? n / 16u : 0;
         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
         const u32x4* dv = (const u32x4*)d;
+        ScanAcc acc = {{0xFFFF, 0xFFFF}, {0, 0}, {0xFFFF, 0xFFFF}, {0, 0}};
         uint64_t c = t0;
-        for (; c + stride < nv; c += 2 * stride) {
-            const u32x4 a = __builtin_nontemporal_load(dv + c);
-            const u32x4 b = __builtin_nontemporal_load(dv + c + stride);
-            minmax_word(a.x, v); minmax_word(a.y, v); minmax_word(a.z, v); minmax_word(a.w, v);
-            minmax_word(b.x, v); minmax_word(b.y, v); minmax_word(b.z, v); minmax_word(b.w, v);
+        for (; c + (kScanU - 1) * stride < nv; c += kScanU * stride) {
+            u32x4 q[kScanU];
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) q[u] = __builtin_nontemporal_load(dv + c + u * stride);
+#pragma unroll
+            for (int u = 0; u < kScanU; ++u) {
+                scan_word(q[u].x, acc);
+                scan_word(q[u].y, acc);
+                scan_word(q[u].z, acc);
+                scan_word(q[u].w, acc);
+            }
         }
-        if (c < nv) {
+        for (; c < nv; c += stride) {
             const u32x4 a = __builtin_nontemporal_load(dv + c);
-            minmax_word(a.x, v); minmax_word(a.y, v); minmax_word(a.z, v); minmax_word(a.w, v);
+            scan_word(a.x, acc);
+            scan_word(a.y, acc);
+            scan_word(a.z, acc);
+            scan_word(a.w, acc);
+        }
+        if (nv > 0) {  // 0xFFFF / 0 where a thread took no chunk: no value is above / below them
+            v[12] = min(v[12], (unsigned long long)min(acc.mn16.x, acc.mn16.y));
+            v[13] = max(v[13], (unsigned long long)max(acc.mx16.x, acc.mx16.y));
+            v[14] = min(v[14], (unsigned long long)min(acc.mn8.x, acc.mn8.y));
+            v[15] = max(v[15], (unsigned long long)max(acc.mx8.x, acc.mx8.y));
         }
         for (uint64_t i = 8u * nv + t0; i < count16; i += stride) {
             const uint32_t x = u16(i);
@@ -258,7 +285,7 @@ hipError_t launch_analyze_stats(const uint8_t* d, uint64_t n, uint64_t n1, uint6
                                 bool scan, AnalyzeRow* rows, AnalyzeStats* out, hipStream_t st) {
     uint64_t blocks = kAnalyzeMinBlocks;
     if (scan) {
-        const uint64_t want = (n / 32u + kThreads - 1) / kThreads;  // two 16-byte chunks per thread
+        const uint64_t want = (n / (16u * kScanU) + kThreads - 1) / kThreads;  // one step of kScanU chunks per thread
         blocks = want < blocks ? blocks : want > (uint64_t)kAnalyzeMaxBlocks ? kAnalyzeMaxBlocks : want;
     }
     hipLaunchKernelGGL(k_analyze, dim3((unsigned)blocks), dim3(kThreads), 0, st, d, n, n1, n2, p, scan ? 1 : 0, rows);
