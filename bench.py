@@ -127,17 +127,38 @@ def binding_roofline(hbm, valu):
     return {"roofline": hbm, "valu_roofline": valu}
 
 
+_VALU_MEASURED = []
+
+
+def measured_valu_peak():
+    """Lane-operations per second this GPU's VALU sustains (the library's
+    issue-rate probe, measured once per run), or None without a GPU."""
+    if not _VALU_MEASURED:
+        try:
+            import imop_amd
+            _VALU_MEASURED.append(imop_amd.valu_issue_rate() * 64)
+        except Exception:  # no GPU / no library: the clock-derived peak only
+            _VALU_MEASURED.append(None)
+    return _VALU_MEASURED[0]
+
+
 def valu_roofline(name, seconds_per_frame, key=None):
     """The VALU roofline object of a workload from its committed SQ counters,
-    or None when no summary holds them."""
+    or None when no summary holds them.  frac is against the clock-derived
+    peak; GPUs of this model sustain different clocks, so the fraction of
+    the issue rate measured on this GPU is given beside it."""
     rel = VALU_PROFILES.get(name)
     insts = _profile_value(rel, key or f"valu_wave_insts_per_frame[{name}]")
     if insts is None:
         return None
     achieved = insts * 64 / seconds_per_frame
-    return {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
-            "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK_LANE_OPS, 4),
-            "valu_wave_insts_per_frame": insts, "counter_source": rel}
+    out = {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
+           "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK_LANE_OPS, 4),
+           "valu_wave_insts_per_frame": insts, "counter_source": rel}
+    m = measured_valu_peak()
+    if m:
+        out.update({"measured_peak": round(m / 1e12, 2), "frac_of_measured_peak": round(achieved / m, 4)})
+    return out
 
 
 def cpu_model():

@@ -235,6 +235,8 @@ def lib() -> ctypes.CDLL:
         L.imop_hip_set_batch_shape.restype = i
         L.imop_hip_get_batch_shape.argtypes = [ctypes.POINTER(i)] * 4
         L.imop_hip_get_batch_shape.restype = i
+        L.imop_hip_valu_issue_rate.argtypes = [ctypes.POINTER(ctypes.c_double), vp]
+        L.imop_hip_valu_issue_rate.restype = i
         _lib = L
     return _lib
 
@@ -386,6 +388,18 @@ def defect_correct_async(d_bayer, width: int, height: int, pattern: int, bit_dep
                                                 int(pattern), bit_depth, ctypes.c_float(threshold), int(max_passes),
                                                 ctypes.c_void_p(d_status.data_ptr()),
                                                 ctypes.c_void_p(_stream_arg(stream)))
+
+
+def valu_issue_rate(stream=None) -> float:
+    """imop_hip_valu_issue_rate: wave64 VALU instructions per second this GPU
+    sustains (a probe kernel); RuntimeError on a HIP failure.  Synchronises
+    the stream."""
+    r = ctypes.c_double()
+    # the null stream unless one is given (the probe synchronises it anyway)
+    e = lib().imop_hip_valu_issue_rate(ctypes.byref(r), ctypes.c_void_p(0 if stream is None else _stream_arg(stream)))
+    if e != 0:
+        raise RuntimeError(f"imop_hip_valu_issue_rate failed: {e}")
+    return r.value
 
 
 def analyze_data(d_data, stream=None) -> tuple[int, dict]:

@@ -410,6 +410,16 @@ int imop_hip_analyze_data(const void* d_data, uint64_t byte_size, imop_hip_data_
 int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, int height, int bit_depth, int is_packed,
                            int* pattern, void* stream);
 
+/* ------------------------------------------------------------ measurement */
+
+/* The vector-ALU issue rate this GPU sustains, in wave64 VALU instructions
+ * per second, from a probe kernel of independent one-cycle-class integer
+ * chains on every SIMD (best of five timed launches after three untimed).
+ * No counterpart in the reference: it calibrates VALU rooflines, since GPUs
+ * of one model run at different sustained clocks.  Synchronises `stream`.  Returns 0, 1 if the pointer is null, 6 on
+ * a HIP failure. */
+int imop_hip_valu_issue_rate(double* wave_instructions_per_second, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
