@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "window_decode.hpp"
 #include "plain.hpp"
 
 #pragma clang fp contract(off)
@@ -48,11 +49,13 @@ __global__ __launch_bounds__(256) void k_amaze(FrameParams f) {
     const int x0 = min((int)(blockIdx.x % ntx) * TW, max(w - TW, 0));
     const int y0 = min((int)(blockIdx.x / ntx) * TH, max(h - TH, 0));
 
-    for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
-        const int j = k / RW, i = k - j * RW;
-        const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
-        s_raw[j][i] = (float)sample_at(f, x, y);
-    }
+    // samples at clamped positions; a window inside the frame needs no clamp
+    if (!window_inside<RW, RH>(f, x0 - 3, y0 - 3, [&](int j, int i, int v) { s_raw[j][i] = (float)v; }))
+        for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
+            const int j = k / RW, i = k - j * RW;
+            const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
+            s_raw[j][i] = (float)sample_at(f, x, y);
+        }
     __syncthreads();
     for (int k = threadIdx.x; k < GH * GW; k += blockDim.x) {
         const int j = k / GW, i = k - j * GW;

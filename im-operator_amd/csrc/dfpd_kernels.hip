@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "window_decode.hpp"
 #include "plain.hpp"
 
 #pragma clang fp contract(off)
@@ -38,11 +39,13 @@ __global__ __launch_bounds__(256) void k_dfpd(FrameParams f) {
     const int ntx = (w + TW - 1) / TW;
     const int x0 = (blockIdx.x % ntx) * TW, y0 = (blockIdx.x / ntx) * TH;
 
-    for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
-        const int j = k / RW, i = k - j * RW;
-        const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
-        s_raw[j][i] = sample_at(f, x, y);
-    }
+    // samples at clamped positions; a window inside the frame needs no clamp
+    if (!window_inside<RW, RH>(f, x0 - 3, y0 - 3, [&](int j, int i, int v) { s_raw[j][i] = v; }))
+        for (int k = threadIdx.x; k < RH * RW; k += blockDim.x) {
+            const int j = k / RW, i = k - j * RW;
+            const int x = min(max(x0 - 3 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
+            s_raw[j][i] = sample_at(f, x, y);
+        }
     __syncthreads();
     for (int k = threadIdx.x; k < GH * GW; k += blockDim.x) {
         const int j = k / GW, i = k - j * GW;

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "window_decode.hpp"
 #include "plain.hpp"
 
 namespace imop_hip {
@@ -88,11 +89,13 @@ __global__ __launch_bounds__(256) void k_l7(FrameParams f) {
     __shared__ __attribute__((aligned(16))) uint8_t o[SH][SW * 6];  // RGB bytes of each row segment
     const int w = f.w, h = f.h;
     const int x0 = blockIdx.x * SW, y0 = blockIdx.y * SH;
-    for (int k = threadIdx.x; k < LH * LW; k += blockDim.x) {
-        const int j = k / LW, i = k - j * LW;
-        const int x = min(max(x0 - 4 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
-        s[j][i] = (uint16_t)sample_at(f, x, y);
-    }
+    // samples at clamped positions; a window inside the frame needs no clamp
+    if (!window_inside<LW, LH>(f, x0 - 4, y0 - 3, [&](int j, int i, int v) { s[j][i] = (uint16_t)v; }))
+        for (int k = threadIdx.x; k < LH * LW; k += blockDim.x) {
+            const int j = k / LW, i = k - j * LW;
+            const int x = min(max(x0 - 4 + i, 0), w - 1), y = min(max(y0 - 3 + j, 0), h - 1);
+            s[j][i] = (uint16_t)sample_at(f, x, y);
+        }
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

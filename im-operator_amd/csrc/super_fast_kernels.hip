@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "device_common.hpp"
+#include "window_decode.hpp"
 #include "plain.hpp"
 
 namespace imop_hip {
@@ -36,11 +37,13 @@ __global__ __launch_bounds__(256) void k_super_fast(FrameParams f) {
     const int w = f.w, h = f.h;
     const int x0 = blockIdx.x * SW, y0 = blockIdx.y * SH;
     static_assert(SH * 64 == 256, "one wave per row");
-    for (int k = threadIdx.x; k < LH * LW; k += blockDim.x) {
-        const int j = k / LW, i = k - j * LW;
-        const int x = min(max(x0 - 1 + i, 0), w - 1), y = min(max(y0 - 1 + j, 0), h - 1);
-        s[j][i] = (uint16_t)sample_at(f, x, y);
-    }
+    // samples at clamped positions; a window inside the frame needs no clamp
+    if (!window_inside<LW, LH>(f, x0 - 1, y0 - 1, [&](int j, int i, int v) { s[j][i] = (uint16_t)v; }))
+        for (int k = threadIdx.x; k < LH * LW; k += blockDim.x) {
+            const int j = k / LW, i = k - j * LW;
+            const int x = min(max(x0 - 1 + i, 0), w - 1), y = min(max(y0 - 1 + j, 0), h - 1);
+            s[j][i] = (uint16_t)sample_at(f, x, y);
+        }
     __syncthreads();
 
     const int row = threadIdx.x >> 6, lane = threadIdx.x & 63;
