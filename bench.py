@@ -99,9 +99,14 @@ def measured_traffic(name):
 
 # Kernels bound by vector-ALU issue rather than HBM: rocprofv3 summaries
 # holding SQ_INSTS_VALU per frame (wave64 instructions, every kernel of the
-# frame summed).  The VALU roofline: each wave64 instruction occupies a
-# 16-lane SIMD for 4 cycles, so the part issues at most 1,024 SIMDs x 16
-# lanes x 2.4 GHz = 39.3 T lane-operations per second.
+# frame summed).  The VALU roofline's peak is MI355X's published FP32 vector
+# rate, 157.3 TFLOP/s = 78.6 T lane-operations per second at 2 FLOP per FMA
+# (1,024 SIMDs x 32 lane-operations x 2.4 GHz, reached by packed FMA).  The
+# single rate, 16 lanes per clock (39.3 T lane-op/s), is what FMA and integer
+# instructions issue at (measured with probe kernels of independent chains);
+# two-operand FP32 add / max chains measured 1.7 times it, so a kernel's
+# ceiling lies between the two with its instruction mix, and the fraction of
+# the single rate is given beside the roofline's.
 VALU_PROFILES = {
     "prism10": "profiles/prism10_trace_traffic_counters.md",
     "rcd10": "profiles/demosaic_algorithms_12mp_valu_counters.md",
@@ -113,7 +118,8 @@ VALU_PROFILES = {
     "lens12": "profiles/k_lens_shading_12mp_u16_trace_traffic.md",
     "chain8": "profiles/chain_8x1006x758_trace_and_traffic.md",
 }
-VALU_PEAK_LANE_OPS = 1024 * 16 * 2.4e9
+VALU_PEAK_LANE_OPS = 1024 * 32 * 2.4e9         # published FP32 vector rate
+VALU_SINGLE_RATE_LANE_OPS = 1024 * 16 * 2.4e9  # FMA / integer / three-operand ops
 
 
 def binding_roofline(hbm, valu):
@@ -130,9 +136,10 @@ def binding_roofline(hbm, valu):
 _VALU_MEASURED = []
 
 
-def measured_valu_peak():
-    """Lane-operations per second this GPU's VALU sustains (the library's
-    issue-rate probe, measured once per run), or None without a GPU."""
+def measured_valu_single_rate():
+    """Single-rate lane-operations per second this GPU's VALU sustains (the
+    library's issue-rate probe of integer chains, measured once per run), or
+    None without a GPU."""
     if not _VALU_MEASURED:
         try:
             import imop_amd
@@ -144,9 +151,11 @@ def measured_valu_peak():
 
 def valu_roofline(name, seconds_per_frame, key=None):
     """The VALU roofline object of a workload from its committed SQ counters,
-    or None when no summary holds them.  frac is against the clock-derived
-    peak; GPUs of this model sustain different clocks, so the fraction of
-    the issue rate measured on this GPU is given beside it."""
+    or None when no summary holds them: frac against the published FP32
+    vector rate, and beside it the fraction of the single (FMA / integer)
+    rate, clock-derived and as measured on this GPU (boxes of this model
+    sustain different clocks).  A kernel whose VALU work is mostly
+    two-operand FP32 operations can exceed the single rate."""
     rel = VALU_PROFILES.get(name)
     insts = _profile_value(rel, key or f"valu_wave_insts_per_frame[{name}]")
     if insts is None:
@@ -154,10 +163,12 @@ def valu_roofline(name, seconds_per_frame, key=None):
     achieved = insts * 64 / seconds_per_frame
     out = {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
            "unit": "T lane-op/s", "frac": round(achieved / VALU_PEAK_LANE_OPS, 4),
+           "single_rate_peak": round(VALU_SINGLE_RATE_LANE_OPS / 1e12, 2),
+           "frac_of_single_rate": round(achieved / VALU_SINGLE_RATE_LANE_OPS, 4),
            "valu_wave_insts_per_frame": insts, "counter_source": rel}
-    m = measured_valu_peak()
+    m = measured_valu_single_rate()
     if m:
-        out.update({"measured_peak": round(m / 1e12, 2), "frac_of_measured_peak": round(achieved / m, 4)})
+        out.update({"measured_single_rate": round(m / 1e12, 2), "frac_of_measured_single_rate": round(achieved / m, 4)})
     return out
 
 
