@@ -1,10 +1,9 @@
-// valu_probe.hip -- the vector-ALU issue rate this GPU sustains, for the
-// bench's VALU rooflines: boxes of one model run at different sustained
-// clocks, so a fraction of the clock-derived peak (1,024 SIMDs x 2.4 GHz / 4
-// cycles per wave64 instruction) can read above 1 on a fast one.  The probe
-// runs eight independent chains of one-cycle-class integer operations per
-// lane (v_xad_u32: xor then add), enough waves to keep every SIMD issuing,
-// and reports wave64 VALU instructions per second.
+// valu_probe.hip -- the single-rate vector-ALU issue rate this GPU sustains
+// (the rate of FMA and integer instructions, 16 lanes per clock per SIMD),
+// for VALU rooflines: boxes of one model run at different sustained clocks.
+// The probe runs eight independent chains of integer operations per lane
+// (v_xad_u32: xor then add), enough waves to keep every SIMD issuing, and
+// reports wave64 VALU instructions per second.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

@@ -412,9 +412,10 @@ int imop_hip_guess_pattern(const void* d_data, uint64_t byte_size, int width, in
 
 /* ------------------------------------------------------------ measurement */
 
-/* The vector-ALU issue rate this GPU sustains, in wave64 VALU instructions
- * per second, from a probe kernel of independent one-cycle-class integer
- * chains on every SIMD (best of five timed launches after three untimed).
+/* The single-rate vector-ALU issue rate this GPU sustains (the rate of FMA
+ * and integer instructions), in wave64 VALU instructions per second, from a
+ * probe kernel of independent integer chains on every SIMD (best of five
+ * timed launches after three untimed).
  * No counterpart in the reference: it calibrates VALU rooflines, since GPUs
  * of one model run at different sustained clocks.  Synchronises `stream`.  Returns 0, 1 if the pointer is null, 6 on
  * a HIP failure. */
