@@ -1584,6 +1584,33 @@ def main():
             "eager_python_loop_us_per_launch": round(eager_ms / args.steps * 1e3, 3),
         },
     }
+    # The same launches over 16 rotating sets (1.46 GB for the 12 MP frame):
+    # above about 1 GB of rotating buffers every launch of the kernel takes
+    # longer (a step between 8 and 16 sets), so the rate at the footprint a
+    # large batch has is reported beside the headline's.
+    big_sets = 16
+    if args.sets < big_sets and not args.no_batch:
+        more_in = [torch.from_numpy(raw.copy()).to(dev) for _ in range(big_sets - args.sets)]
+        more_out = [torch.empty(out_b, dtype=torch.uint8, device=dev) for _ in range(big_sets - args.sets)]
+        bins, bouts = d_in + more_in, d_out + more_out
+        g16 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g16):
+            for k in range(args.steps):
+                if imop_amd.demosaic(bins[k % big_sets], bouts[k % big_sets], w, h, 0, algo, bd, packed, blv) != 0:
+                    raise RuntimeError("imop_hip_demosaic failed under capture")
+        for _ in range(5):
+            g16.replay()
+        torch.cuda.synchronize()
+        start.record(stream)
+        g16.replay()
+        end.record(stream)
+        torch.cuda.synchronize()
+        big_us = start.elapsed_time(end) / args.steps * 1e3
+        result["roofline"]["large_footprint"] = {
+            "buffer_sets": big_sets, "footprint_bytes": big_sets * alg_bytes, "mean_launch_us": round(big_us, 3),
+            "achieved": round(alg_bytes / big_us / 1e3, 1), "frac": round(alg_bytes / big_us / 1e3 / HBM_PEAK_GBS, 4)}
+        log(f"bench: {big_sets} rotating sets: mean launch {big_us:.2f} us")
+        del g16, more_in, more_out, bins, bouts
     calibration_copies(args.copy_mb)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline")

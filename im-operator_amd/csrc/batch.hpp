@@ -21,7 +21,7 @@ struct BatchFrame {
     int32_t flags;         // bit 0: RGB as u16; bit 1: black level applied
     int32_t bl_i[4];       // integral black level by (y & 1) * 2 + (x & 1); 0 without
     uint16_t nsx, nby;     // strips (strip_count) and group rows (rows per wave x waves per workgroup)
-    uint32_t first;        // index of the frame's first work item in the batch
+    uint32_t first;        // the frame's first work item in the batch (frame-major placement: its first slot)
     uint64_t place_mul;    // place_reciprocal(nsx * nby, strip-fastest ? nsx : nby)
 };
 static_assert(sizeof(BatchFrame) == 80, "BatchFrame layout");
@@ -32,14 +32,27 @@ static_assert(sizeof(BatchFrame) == 80, "BatchFrame layout");
 // end belongs to fi and one at or past it to a later frame.
 constexpr int kBatchChunk = 32;
 
+// Frame-major placement of one frame of n items: slot j (counted from the
+// frame's first slot, j < ceil(n / 8)) of XCD xcd runs item `local`, where
+// XCD x owns the contiguous run of q + (x < r) items (n = 8q + r) after
+// those of XCDs 0..x-1 and XCDs 4-7 walk their runs from the end -- the
+// single-frame kernel's placement of a launch of n workgroups.  False for
+// the idle slot of an XCD whose run is one short.
+__host__ __device__ inline bool frame_major_local(uint32_t n, uint32_t xcd, uint32_t j, uint32_t& local) {
+    const uint32_t q = n >> 3, r = n & 7u, cnt = q + (xcd < r ? 1u : 0u);
+    if (j >= cnt) return false;
+    local = xcd * q + (xcd < r ? xcd : r) + (xcd >= 4 ? cnt - 1u - j : j);
+    return true;
+}
+
 // Kernel arguments of one batched launch.
 struct BatchArgs {
     const BatchFrame* frames;  // device table, sorted by `first`
     const uint2* chunk;        // device table of ceil(total / kBatchChunk) entries {fi, end}
     int32_t nframes;
-    uint32_t total;            // work items (frame, strip, group row) of all frames
+    uint32_t total;            // work items (frame, strip, group row) of all frames; frame-major: slots
     int32_t wpb;               // 4: group items (4 waves down a strip); 1: one wave per item
-    int32_t order;             // bit 1: strip-fastest within a frame
+    int32_t order;             // bit 1: strip-fastest within a frame; bit 2: frame-major placement
 };
 
 // Enqueue k_batch for one layout class -- pattern, alg (0 HQLI, 1 MG), raw

@@ -15,12 +15,16 @@
 // 54-64 VGPRs, 8 waves per SIMD, where one inlining all eight classes needs
 // 91-123 (4-5 waves per SIMD) and ran the 64-frame mix 1.9x slower.
 //
-// Schedule.  Workgroups are dealt round-robin to the 8 XCDs; the list is cut
-// into 8 contiguous slices, one per XCD, so groups that share raw rows run
-// on the same XCD (one L2), and XCDs 4-7 take their slices from the end, as
-// the single-frame kernel does, so the last slice's final rows do not finish
-// the launch.  PERSIST = false: one workgroup per item, placed as the
-// single-frame kernel places its groups.  PERSIST = true: a grid of a few
+// Schedule.  Workgroups are dealt round-robin to the 8 XCDs.  Frame-major
+// placement (order bit 2, the default; one workgroup per item): each frame
+// is cut into 8 contiguous runs, one per XCD, exactly as its single-frame
+// launch places its groups, and the frames follow one another through the
+// grid (frame_major_item).  Without it the whole list is cut into 8
+// contiguous slices, one per XCD, so that the XCDs work on frames up to 1.4
+// frames apart (the 64-frame mix ran 11 % slower so).  Either way groups
+// that share raw rows run on the same XCD (one L2), and XCDs 4-7 walk their
+// runs from the end so the last run's final rows do not finish the launch.
+// PERSIST = false: one workgroup per item.  PERSIST = true: a grid of a few
 // workgroups per CU walks each slice with a stride of that slice's workgroup
 // count (its loop keeps 13 more VGPRs live, 6 waves per SIMD).  Each item's
 // frame comes from a chunk table (item_frame) by scalar loads, and its strip
@@ -63,12 +67,12 @@ __device__ __forceinline__ int item_frame(const BatchArgs& a, uint32_t it) {
     return fi;
 }
 
-// One work item `it` of a batch launch, as wave w of the workgroup.
+// Item `local` (frame-local index) of frame bf, as wave w of the workgroup.
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
-__device__ __forceinline__ void batch_item(const BatchArgs& a, uint32_t it, int w, uint8_t* L) {
+__device__ __forceinline__ void batch_item_at(const BatchArgs& a, const BatchFrame& bf, uint32_t local, int w,
+                                              uint8_t* L) {
     constexpr int GB = 4;
     using G = GroupLayout<TH, GB>;
-    const BatchFrame& bf = a.frames[item_frame(a, it)];
     FastParams f{};
     f.in = load_global_ptr(&bf.in);
     f.out = load_global_ptr(&bf.out);
@@ -89,8 +93,31 @@ __device__ __forceinline__ void batch_item(const BatchArgs& a, uint32_t it, int 
     f.nsx = nsx;
     f.nby = nby;
     int sx, gy;
-    place_split(it - bf.first, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, gy);
+    place_split(local, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, gy);
     group_item<FMT, PAT, MODE, TH, GB, ODD, false, false>(f, strip_of(f, sx), gy * G::kRows, w, L, nullptr);
+}
+
+// One work item `it` of a batch launch (item-major list), as wave w.
+template <int FMT, int PAT, int MODE, int TH, bool ODD>
+__device__ __forceinline__ void batch_item(const BatchArgs& a, uint32_t it, int w, uint8_t* L) {
+    const BatchFrame& bf = a.frames[item_frame(a, it)];
+    batch_item_at<FMT, PAT, MODE, TH, ODD>(a, bf, it - bf.first, w, L);
+}
+
+// Frame-major placement (order bit 2, one workgroup per item): workgroup b
+// is slot b >> 3 of XCD b & 7.  Slots are numbered frame by frame
+// (BatchFrame::first counts slots, ceil(items / 8) a frame), and in its
+// slots of a frame XCD x runs the x-th contiguous eighth of its items
+// (frame_major_local): every frame is placed as one single-frame launch
+// places its groups, and the frames follow one another through the grid.
+// Returns the frame (null for the at most 7 idle workgroups of a frame whose
+// item count is not a multiple of 8) and its item in `local`.
+__device__ __forceinline__ const BatchFrame* frame_major_item(const BatchArgs& a, uint32_t& local) {
+    const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x >> 3));
+    if (s >= a.total) return nullptr;
+    const BatchFrame& bf = a.frames[item_frame(a, s)];
+    const uint32_t n = (uint32_t)bf.nsx * (uint32_t)bf.nby;
+    return frame_major_local(n, blockIdx.x & 7u, s - bf.first, local) ? &bf : nullptr;
 }
 
 template <int FMT, int PAT, int MODE, int TH, bool ODD, bool PERSIST>
@@ -100,6 +127,12 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
     uint8_t* L = (uint8_t*)s_group;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int nblk = gridDim.x;
+    if (!PERSIST && (a.order & 4)) {
+        uint32_t local = 0;
+        const BatchFrame* bf = frame_major_item(a, local);
+        if (bf) batch_item_at<FMT, PAT, MODE, TH, ODD>(a, *bf, local, w, L);
+        return;
+    }
     if (!PERSIST) {
         // one workgroup per item (the host launches `total` workgroups): the
         // single-frame kernel's bijection, XCD x running the contiguous run
@@ -131,8 +164,8 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
 // kernel's per-wave path (run_band); no LDS is shared and no barrier taken.
 // Workgroups of one wave, placed like the group form's workgroups.
 template <int FMT, int PAT, int MODE, int TH, bool ODD>
-__device__ __forceinline__ void batch_wave_item(const BatchArgs& a, uint32_t it, uint32_t* stage) {
-    const BatchFrame& bf = a.frames[item_frame(a, it)];
+__device__ __forceinline__ void batch_wave_item_at(const BatchArgs& a, const BatchFrame& bf, uint32_t local,
+                                                   uint32_t* stage) {
     FastParams f{};
     f.in = load_global_ptr(&bf.in);
     f.out = load_global_ptr(&bf.out);
@@ -153,7 +186,7 @@ __device__ __forceinline__ void batch_wave_item(const BatchArgs& a, uint32_t it,
     f.nsx = nsx;
     f.nby = nby;
     int sx, by;
-    place_split(it - bf.first, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, by);
+    place_split(local, (a.order & 2) != 0, (uint32_t)nsx, (uint32_t)nby, bf.place_mul, sx, by);
     const Strip sp = strip_of(f, sx);
     const int y0 = by * TH;
     const bool vrim = y0 < 3 || y0 + TH + 3 > f.h - 1;
@@ -163,11 +196,23 @@ __device__ __forceinline__ void batch_wave_item(const BatchArgs& a, uint32_t it,
     else run_band<FMT, PAT, MODE, false, TH, ODD, false, false>(f, sp, y0, stage);
 }
 
+template <int FMT, int PAT, int MODE, int TH, bool ODD>
+__device__ __forceinline__ void batch_wave_item(const BatchArgs& a, uint32_t it, uint32_t* stage) {
+    const BatchFrame& bf = a.frames[item_frame(a, it)];
+    batch_wave_item_at<FMT, PAT, MODE, TH, ODD>(a, bf, it - bf.first, stage);
+}
+
 template <int FMT, int PAT, int MODE, int TH, bool ODD, bool PERSIST>
 __global__ __launch_bounds__(64) void k_batch_w(BatchArgs a) {
     __shared__ uint4 s_stage[128];  // one 1536-byte row slice plus the chunk reads' reach
     uint32_t* stage = (uint32_t*)s_stage;
     const int nblk = gridDim.x;
+    if (!PERSIST && (a.order & 4)) {
+        uint32_t local = 0;
+        const BatchFrame* bf = frame_major_item(a, local);
+        if (bf) batch_wave_item_at<FMT, PAT, MODE, TH, ODD>(a, *bf, local, stage);
+        return;
+    }
     if (!PERSIST) {
         const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
         const int q8 = nblk >> 3, r8 = nblk & 7;

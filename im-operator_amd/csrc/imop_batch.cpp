@@ -67,8 +67,9 @@ namespace {
 // Launch shape of k_batch (imop_hip_set_batch_shape): waves per workgroup
 // (4: group items of 4 waves x rows, rows 2 or 4; 1: wave items of rows, 4 or
 // 8), workgroups per CU of a persistent grid (0: one workgroup per item),
-// order (bit 1: strip-fastest).
-std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{2};
+// order (bit 1: strip-fastest within a frame; bit 2: frame-major placement,
+// one-workgroup-per-item grids only).
+std::atomic<int> g_rows{4}, g_wpb{4}, g_bpc{0}, g_order{6};
 
 // Rows per wave the form runs: group items take 2 or 4, wave items 4 or 8.
 int batch_rows(int rows, int wpb) { return wpb == 1 ? (rows == 8 ? 8 : 4) : (rows == 2 ? 2 : 4); }
@@ -93,8 +94,8 @@ struct Group {
     int pattern = 0, alg = 0, fmt = 0;
     bool odd = false;
     std::vector<BatchFrame> frames;
-    uint32_t total = 0;
-    std::vector<uint32_t> chunk;  // {frame of item c * kBatchChunk, first item of the next frame} by chunk c
+    uint32_t total = 0;           // items, or slots under frame-major placement (BatchFrame::first likewise)
+    std::vector<uint32_t> chunk;  // {frame of entry c * kBatchChunk, first entry of the next frame} by chunk c
 
     // The device copy: the frame table, then the chunk table at a 16-byte boundary.
     size_t chunk_offset() const { return (frames.size() * sizeof(BatchFrame) + 15u) & ~(size_t)15u; }
@@ -116,7 +117,7 @@ struct Prepared {
     std::vector<Group> groups;
     std::vector<Single> singles;
     size_t prism_ws = 0;
-    int rows = 4, wpb = 4, bpc = 0, order = 2;
+    int rows = 4, wpb = 4, bpc = 0, order = 6;
 };
 
 int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
@@ -133,7 +134,9 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
     P.wpb = g_wpb;
     P.rows = batch_rows(g_rows, P.wpb);
     P.bpc = g_bpc;
-    P.order = g_order;
+    // frame-major placement is a form of the one-workgroup-per-item grid
+    P.order = g_order & (P.bpc > 0 ? 2 : 6);
+    const bool frame_major = (P.order & 4) != 0;
     std::map<int, size_t> index;  // group of each launch key
     for (int i = 0; i < n; ++i) {
         const imop_hip_frame_desc& d = fr[i];
@@ -189,14 +192,20 @@ int prepare(const imop_hip_frame_desc* fr, int n, Prepared& P, int* failed) {
         b.first = g.total;
         b.place_mul = place_reciprocal(items, (P.order & 2) ? (uint64_t)nsx : (uint64_t)nby);
         g.frames.push_back(b);
-        g.total += (uint32_t)items;
+        // frame-major: the frame takes ceil(items / 8) slots of 8 workgroups
+        g.total += frame_major ? (uint32_t)((items + 7) / 8) : (uint32_t)items;
+    }
+    for (const Group& g : P.groups) {
+        // grid of one workgroup per item (slot) fits the launch limits
+        const uint64_t blocks = frame_major ? 8ull * g.total : (uint64_t)g.total;
+        if (P.bpc == 0 && blocks * (P.wpb == 1 ? 64u : 256u) >= (1ull << 32)) return kInternal;
     }
     for (Group& g : P.groups) {
         const uint32_t nchunks = (g.total + kBatchChunk - 1) / kBatchChunk;
         g.chunk.resize(2 * (size_t)nchunks);
         size_t fi = 0;
         for (uint32_t c = 0; c < nchunks; ++c) {
-            const uint32_t item = c * (uint32_t)kBatchChunk;
+            const uint32_t item = c * (uint32_t)kBatchChunk;  // an item, or a slot
             while (fi + 1 < g.frames.size() && g.frames[fi + 1].first <= item) ++fi;
             g.chunk[2 * c] = (uint32_t)fi;
             g.chunk[2 * c + 1] = fi + 1 < g.frames.size() ? g.frames[fi + 1].first : g.total;
@@ -219,7 +228,8 @@ int enqueue(const Prepared& P, const std::vector<void*>& tables, void* prism_ws,
         a.total = g.total;
         a.wpb = P.wpb;
         a.order = P.order;
-        int64_t blocks = (int64_t)g.total;  // one workgroup per item
+        // one workgroup per item, or 8 per slot (frame-major)
+        int64_t blocks = (P.order & 4) ? 8 * (int64_t)g.total : (int64_t)g.total;
         if (P.bpc > 0 && cus > 0) blocks = std::min<int64_t>(blocks, (int64_t)P.bpc * cus);
         if (launch_batch(a, g.pattern, g.alg, g.fmt, g.odd, P.rows, P.bpc > 0, (int)blocks, st) != hipSuccess)
             return kInternal;
@@ -363,7 +373,7 @@ int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_
     if (rows_per_wave == 2 || rows_per_wave == 4 || rows_per_wave == 8) g_rows = rows_per_wave;
     if (waves_per_block == 1 || waves_per_block == 4) g_wpb = waves_per_block;
     if (blocks_per_cu >= 0 && blocks_per_cu <= 64) g_bpc = blocks_per_cu;
-    if (order >= 0 && order <= 3) g_order = order & 2;
+    if (order >= 0 && order <= 7) g_order = order & 6;
     return kOk;
 }
 

@@ -87,9 +87,10 @@ def tune(w: int, h: int, bd: int, packed: bool, algo: int, black_level=None, see
 
 
 # (rows, waves per workgroup) forms: group items of 4 waves x 2 / 4 rows, wave
-# items of 4 / 8 rows; workgroups per CU 0 (one per item) or persistent
+# items of 4 / 8 rows; workgroups per CU 0 (one per item) or persistent; order
+# bit 1 strip-fastest, bit 2 frame-major placement (one-per-item grids only)
 BATCH_CANDIDATES = [(r, w, b, o) for (r, w) in ((2, 4), (4, 4), (4, 1), (8, 1)) for b in (0, 4, 8, 32)
-                    for o in (0, 2)]
+                    for o in ((0, 2, 4, 6) if b == 0 else (0, 2))]
 
 
 def tune_batch(desc_sets, digest=None, candidates=BATCH_CANDIDATES, iters: int = 8, verbose: bool = True):

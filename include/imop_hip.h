@@ -191,8 +191,11 @@ int imop_hip_batch_plan_destroy(imop_hip_batch_plan* plan);
  * value the form does not take runs as its nearest (group form: 8 -> 4;
  * wave form: 2 -> 4).  blocks_per_cu: workgroups per CU of the persistent
  * grid (1..64; 0 = one workgroup per item); order: bit 1 strip-fastest
- * within a frame.  Initial shape (4, 4, 0, 2).  Out-of-range values leave a
- * setting unchanged.  Process-wide; no counterpart in the reference. */
+ * within a frame, bit 2 frame-major placement (one-workgroup-per-item grids:
+ * the frames run one after another, each spread over the 8 XCDs as a
+ * single-frame launch spreads it; else the whole list is cut into 8
+ * contiguous per-XCD runs).  Initial shape (4, 4, 0, 6).  Out-of-range values
+ * leave a setting unchanged.  Process-wide; no counterpart in the reference. */
 int imop_hip_set_batch_shape(int rows_per_wave, int waves_per_block, int blocks_per_cu, int order);
 int imop_hip_get_batch_shape(int* rows_per_wave, int* waves_per_block, int* blocks_per_cu, int* order);
 

@@ -106,7 +106,7 @@ def test_ragged_batch_plan_replays_match_oracle():
 
 @pytest.mark.parametrize("rows,wpb", [(2, 4), (4, 4), (4, 1), (8, 1)])
 @pytest.mark.parametrize("bpc", [0, 1, 3, 16])
-@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("order", [0, 2, 4, 6])
 def test_every_batch_shape(rows, wpb, bpc, order, restore_batch_shape):
     imop_amd.set_batch_shape(rows, wpb, bpc, order)
     assert imop_amd.get_batch_shape() == (rows, wpb, bpc, order)
@@ -116,6 +116,31 @@ def test_every_batch_shape(rows, wpb, bpc, order, restore_batch_shape):
     torch.cuda.synchronize()
     assert (e, failed) == (0, -1)
     _check_against_oracle(frames)
+
+
+@pytest.mark.parametrize("wpb", [4, 1])
+def test_frame_major_small_frames_cover_every_item(wpb, restore_batch_shape):
+    """Frame-major placement over frames of 1 to 23 items (fewer than, equal
+    to and not multiples of the 8 XCDs), so every frame leaves some XCDs idle
+    slots, between larger ones; eager and as a replayed plan."""
+    imop_amd.set_batch_shape(4, wpb, 0, 6)
+    sizes = [(256, 16), (224, 8), (1006, 16), (256, 64), (512, 40), (2000, 24), (1006, 758), (480, 52), (256, 16)]
+    specs = [dict(w=w, h=h, bd=[8, 10, 12][k % 3], packed=k % 3 != 0, algo=MG if k % 2 else HQLI, pattern=k % 4,
+                  bl=None if k % 2 else [16.0] * 4, out_elem=2, seed=40 + k) for k, (w, h) in enumerate(sizes)]
+    frames = _materialise(specs)
+    print(f"frame-major small frames wpb {wpb}", flush=True)
+    e, failed = imop_amd.demosaic_batch(_descs(frames))
+    torch.cuda.synchronize()
+    assert (e, failed) == (0, -1)
+    _check_against_oracle(frames)
+    with imop_amd.BatchPlan(_descs(frames)) as plan:
+        assert (plan.error, plan.failed_frame) == (0, -1)
+        for rep in range(2):
+            for _, _, _, d_out in frames:
+                d_out.fill_(0x3C)
+            assert plan.launch() == 0
+            torch.cuda.synchronize()
+            _check_against_oracle(frames)
 
 
 def test_tiny_grid_covers_every_item(restore_batch_shape):
